@@ -1,0 +1,24 @@
+/* psst_oracle.h -- TEST INFRASTRUCTURE ONLY: C API of the CPU restatement of
+ * the reference solver (psst_oracle.cpp).  Loaded by tests/, smoke() and
+ * bench.py's CPU-baseline leg; never by the product library. */
+#ifndef PSST_ORACLE_H
+#define PSST_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+const char *oracle_last_error(void);
+void *oracle_open(const char *mesh_path, const char *case_prefix);
+void oracle_close(void *h);
+int oracle_info(void *h, int *info);          /* dim, N, G, F, rk, iter, max_iter, iter_step */
+int oracle_constants(void *h, double *c);     /* gamma, rm, cp, cv, ref_vel, cfl, min_vol, uniform_dt */
+int oracle_run(void *h, int n_iter, double *res_hist);
+int oracle_get_state(void *h, double *prim);
+int oracle_set_state(void *h, const double *prim);
+int oracle_get_w(void *h, double *w);
+int oracle_gradients(void *h, double *grads);
+int oracle_residual(void *h, double *R);
+int oracle_write_sav(void *h, const char *path, long iter_value);
+#ifdef __cplusplus
+}
+#endif
+#endif
