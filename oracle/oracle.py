@@ -1,0 +1,147 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of ``_build/liboracle_psst.so``,
+the serial CPU restatement of the reference solver (psst_oracle.cpp).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use it,
+as the checker; the product package never imports it."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle_psst.so")
+CLI_PATH = os.path.join(HERE, "_build", "psst_oracle_cli")
+REF_DUMP = os.path.join(HERE, "_ref", "psst_ref_dump")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.oracle_open.restype = C.c_void_p
+        L.oracle_open.argtypes = [C.c_char_p, C.c_char_p]
+        L.oracle_last_error.restype = C.c_char_p
+        L.oracle_close.argtypes = [C.c_void_p]
+        for n in ("oracle_info",):
+            getattr(L, n).argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        for n in ("oracle_constants", "oracle_get_state", "oracle_set_state", "oracle_get_w", "oracle_gradients",
+                  "oracle_residual"):
+            getattr(L, n).argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.oracle_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+        L.oracle_write_sav.argtypes = [C.c_void_p, C.c_char_p, C.c_long]
+        L.oracle_limiter.restype = C.c_double
+        L.oracle_limiter.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.oracle_face_flux.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    """The reference's explicit_solve restated on the CPU (FP64, serial)."""
+
+    def __init__(self, mesh_path, case_prefix):
+        L = lib()
+        h = L.oracle_open(str(mesh_path).encode(), str(case_prefix).encode())
+        if not h:
+            raise RuntimeError("oracle: " + L.oracle_last_error().decode())
+        self._h = C.c_void_p(h)
+        info = (C.c_int * 8)()
+        L.oracle_info(self._h, info)
+        self.dim, self.n_cells, self.n_ghosts, self.n_faces, self.rk_levels, self.iter, self.max_iter, \
+            self.save_step = list(info)
+        c = np.zeros(8)
+        L.oracle_constants(self._h, _dp(c))
+        self.gamma, self.rm, self.cp, self.cv, self.ref_vel, self.cfl, self.min_vol, self.uniform_dt = c
+
+    def close(self):
+        if self._h:
+            lib().oracle_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n_total(self):
+        return self.n_cells + self.n_ghosts
+
+    def run(self, n_iter):
+        res = np.zeros((n_iter, self.dim + 2))
+        if lib().oracle_run(self._h, int(n_iter), _dp(res)) != 0:
+            raise RuntimeError("oracle: " + lib().oracle_last_error().decode())
+        return res
+
+    def state(self):
+        p = np.zeros((self.dim + 3, self.n_total))
+        lib().oracle_get_state(self._h, _dp(p))
+        return p
+
+    def set_state(self, prim):
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        lib().oracle_set_state(self._h, _dp(prim))
+
+    def conservative(self):
+        w = np.zeros((self.dim + 2, self.n_cells))
+        lib().oracle_get_w(self._h, _dp(w))
+        return w
+
+    def gradients(self):
+        g = np.zeros((self.dim + 2, self.dim, self.n_cells))
+        if lib().oracle_gradients(self._h, _dp(g)) != 0:
+            raise RuntimeError("oracle: " + lib().oracle_last_error().decode())
+        return g
+
+    def residual(self):
+        r = np.zeros((self.dim + 2, self.n_cells))
+        if lib().oracle_residual(self._h, _dp(r)) != 0:
+            raise RuntimeError("oracle: " + lib().oracle_last_error().decode())
+        return r
+
+    def write_sav(self, path, iter_value):
+        lib().oracle_write_sav(self._h, str(path).encode(), int(iter_value))
+
+
+def limiter(kind, num, den):
+    """psi(num, den) of limiter kind 1 (Van Albada), 2 (MinMod), 3 (Van Leer)."""
+    return lib().oracle_limiter(int(kind), float(num), float(den))
+
+
+def face_flux(flux, n, q0, q1, gamma=1.4, rm=287.0):
+    """Interior-face flux (no reconstruction) of states q = (u..., p, T, rho);
+    n is the unit normal pointing toward state 0."""
+    cp = gamma * rm / (gamma - 1.0)
+    cv = cp / gamma
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    q0 = np.ascontiguousarray(q0, dtype=np.float64)
+    q1 = np.ascontiguousarray(q1, dtype=np.float64)
+    dim = n.size
+    F = np.zeros(dim + 2)
+    if lib().oracle_face_flux(int(flux), dim, gamma, rm, cp, cv, _dp(n), _dp(q0), _dp(q1), _dp(F)) != 0:
+        raise RuntimeError("oracle: " + lib().oracle_last_error().decode())
+    return F
+
+
+def read_hex_dump(path, dim):
+    """State dump written by the oracle CLI or the reference dump driver: one %a
+    value per line, per cell u..., p, T, rho -> array [(dim+3), n]."""
+    with open(path) as f:
+        vals = np.array([float.fromhex(l) for l in f if l.strip()])
+    return vals.reshape(-1, dim + 3).T.copy()

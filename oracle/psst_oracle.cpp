@@ -1036,12 +1036,15 @@ void boundary_flux(Oracle &o, const Face &f) {
     }
 }
 
-void interior_flux(Oracle &o, const Face &f) {
-    const int dim = o.dim;
+struct Gas {
+    double gamma, rm, cp, cv;
+};
+
+/* Interior-face Riemann flux of two (reconstructed) states; rv[0] is the
+ * state of neighbour 0 (the side the normal points to), rv[1] of neighbour 1.
+ * HLL psstConvectiveFlow.cpp:215-286, HLLC :499-585, AUSM :782-851. */
+void riemann_flux(int flux, int dim, const Gas &gas, const double *n, const RVar rv[2], double *flow) {
     const int ws = dim + 2;
-    RVar rv[2];
-    reconstruct(o, f, rv);
-    Vol *v[2] = {&o.vols[f.nb[0]], &o.vols[f.nb[1]]};
     double p[2], T[2], rho[2], vel[2][3], vn[2], a[2], H[2], E[2], M[2];
     double fi[2][5], wi[2][5];
     for (int j = 0; j < 2; ++j) {
@@ -1052,17 +1055,17 @@ void interior_flux(Oracle &o, const Face &f) {
         rho[j] = rv[j].rho;
         for (int k = 0; k < dim; ++k) {
             vel[j][k] = rv[j].u[k];
-            vn[j] += vel[j][k] * f.n[k];
+            vn[j] += vel[j][k] * n[k];
             vsq += vel[j][k] * vel[j][k];
         }
-        a[j] = std::sqrt(o.gamma * o.rm * T[j]);
-        H[j] = o.cp * T[j] + 0.5 * vsq;
-        if (o.flux == 1) {
+        a[j] = std::sqrt(gas.gamma * gas.rm * T[j]);
+        H[j] = gas.cp * T[j] + 0.5 * vsq;
+        if (flux == 1) {
             M[j] = vn[j] / a[j];
         } else {
-            E[j] = o.cv * T[j] + 0.5 * vsq;
+            E[j] = gas.cv * T[j] + 0.5 * vsq;
             for (int k = 0; k < dim; ++k) {
-                fi[j][k] = rho[j] * vel[j][k] * vn[j] + p[j] * f.n[k];
+                fi[j][k] = rho[j] * vel[j][k] * vn[j] + p[j] * n[k];
                 wi[j][k] = rho[j] * vel[j][k];
             }
             wi[j][dim] = rho[j];
@@ -1071,17 +1074,15 @@ void interior_flux(Oracle &o, const Face &f) {
             fi[j][dim + 1] = rho[j] * vn[j] * H[j];
         }
     }
-    double flow[5];
-    if (o.flux == 2 || o.flux == 3) {
-        /* HLL :247-286 ; HLLC :531-585 */
+    if (flux == 2 || flux == 3) {
         double Hr = average_by_roe(H[1], H[0], rho[1], rho[0]);
         double vrn = 0.0, vsq = 0.0, vr[3];
         for (int k = 0; k < dim; ++k) {
             vr[k] = average_by_roe(vel[1][k], vel[0][k], rho[1], rho[0]);
-            vrn += vr[k] * f.n[k];
+            vrn += vr[k] * n[k];
             vsq += vr[k] * vr[k];
         }
-        double ar = std::sqrt((o.gamma - 1.0) * (Hr - 0.5 * vsq));
+        double ar = std::sqrt((gas.gamma - 1.0) * (Hr - 0.5 * vsq));
         double s[2], temp;
         s[1] = vn[1] - a[1];
         temp = vrn - ar;
@@ -1089,7 +1090,7 @@ void interior_flux(Oracle &o, const Face &f) {
         s[0] = vn[0] + a[0];
         temp = vrn + ar;
         if (s[0] < temp) s[0] = temp;
-        if (o.flux == 2) {
+        if (flux == 2) {
             if (s[1] >= 0.0) {
                 for (int j = 0; j < ws; ++j) flow[j] = fi[1][j];
             } else if (s[1] <= 0.0 && s[0] >= 0.0) {
@@ -1110,7 +1111,7 @@ void interior_flux(Oracle &o, const Face &f) {
                 temp = s[K] - sh;
                 for (int j = 0; j < dim; ++j)
                     flow[j] = (sh * (s[K] * wi[K][j] - fi[K][j]) +
-                               s[K] * (p[K] + rho[K] * (s[K] - vn[K]) * (sh - vn[K])) * f.n[j]) / temp;
+                               s[K] * (p[K] + rho[K] * (s[K] - vn[K]) * (sh - vn[K])) * n[j]) / temp;
                 flow[dim] = (sh * (s[K] * wi[K][dim] - fi[K][dim])) / temp;
                 flow[dim + 1] = (sh * (s[K] * wi[K][dim + 1] - fi[K][dim + 1]) +
                                  s[K] * (p[K] + rho[K] * (s[K] - vn[K]) * (sh - vn[K])) * sh) / temp;
@@ -1121,7 +1122,6 @@ void interior_flux(Oracle &o, const Face &f) {
             }
         }
     } else {
-        /* AUSM :804-851 */
         double Ms[2], ps[2];
         if (M[1] >= 1.0) { Ms[1] = M[1]; ps[1] = p[1]; }
         else if (M[1] <= -1.0) { Ms[1] = 0.0; ps[1] = 0.0; }
@@ -1140,12 +1140,22 @@ void interior_flux(Oracle &o, const Face &f) {
         int K = (Mf > 0.0) ? 1 : 0;
         double temp = Mf * a[K] * rho[K];
         for (int j = 0; j < dim; ++j) {
-            double pflow = pf * f.n[j];
+            double pflow = pf * n[j];
             flow[j] = temp * vel[K][j] + pflow;
         }
         flow[dim] = temp;
         flow[dim + 1] = temp * H[K];
     }
+}
+
+void interior_flux(Oracle &o, const Face &f) {
+    const int ws = o.dim + 2;
+    RVar rv[2];
+    reconstruct(o, f, rv);
+    Vol *v[2] = {&o.vols[f.nb[0]], &o.vols[f.nb[1]]};
+    double flow[5];
+    Gas gas{o.gamma, o.rm, o.cp, o.cv};
+    riemann_flux(o.flux, o.dim, gas, f.n, rv, flow);
     int sign = 1;
     for (int j = 0; j < 2; ++j) {
         sign = (-1) * sign;
@@ -1517,6 +1527,30 @@ int oracle_residual(void *h, double *R) {
         }
     });
     return 0;
+}
+
+/* Scalar limiter psi(num, den) (psstScalarLimiters.cpp). */
+double oracle_limiter(int kind, double num, double den) { return limiter_value(kind, num, den); }
+
+/* Interior-face flux of two states q = (u..., p, T, rho) for unit normal n
+ * pointing toward state 0; returns 0, or 1 when the reference would throw. */
+int oracle_face_flux(int flux, int dim, double gamma, double rm, double cp, double cv, const double *n,
+                     const double *q0, const double *q1, double *F) {
+    RVar rv[2];
+    const double *q[2] = {q0, q1};
+    for (int j = 0; j < 2; ++j) {
+        for (int k = 0; k < dim; ++k) rv[j].u[k] = q[j][k];
+        rv[j].p = q[j][dim];
+        rv[j].T = q[j][dim + 1];
+        rv[j].rho = q[j][dim + 2];
+    }
+    try {
+        riemann_flux(flux, dim, Gas{gamma, rm, cp, cv}, n, rv, F);
+        return 0;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return 1;
+    }
 }
 
 /* Write the reference's `.sav` format (psstFiniteVolumeGrid.cpp:606-639). */

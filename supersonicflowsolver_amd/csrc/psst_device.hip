@@ -1,0 +1,1461 @@
+// psst_device.hip -- gfx950 kernels and the device context behind the C-ABI
+// in include/psst_gpu.h.
+//
+// One iteration of the reference's explicit_solve (psstFiniteVolumeGrid.cpp:305-397):
+//   k_grad_*      gradients of (u, p, T) once per iteration (second order)
+//   per RK stage:
+//     k_flux      one thread per face: reconstruction + Riemann flux, F*A
+//     k_update    one thread per interior cell: residual R = ordered gather of
+//                 the +-F*A of its faces, time step, RK update, primitive
+//                 recovery, residual max-norm
+//                 (k_update_ordered_*: the reference's in-place sweep for the
+//                 neighbour-dependent time step, level-scheduled)
+//     k_bc        one thread per ghost: boundary state
+// Data layout: structure of arrays in HBM.  Geometry is FP64 always; the
+// state type T is double (parity) or float (FP32 mode).  Prim buffers are
+// ping-ponged so a stage reads its start state while writing the new one.
+//
+// Bitwise parity with the reference in FP64 rests on: the same association
+// of every expression (each kernel function names the lines it restates),
+// no FMA contraction (-ffp-contract=off), IEEE-correct division and square
+// root, and sums taken in the reference's order (per-cell CSR lists; no
+// floating-point atomics).  The residual max-norm is order-free and uses
+// integer atomics on the bit patterns of non-negative doubles.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "psst_gpu.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kResSlots = 64;  // residual max-norm is spread over this many slots per column
+
+// ---------------------------------------------------------------------------
+// Device pointer bundles
+// ---------------------------------------------------------------------------
+struct Geom {
+    int N, G, NT, F;
+    const double *__restrict__ vol;   // [NT]
+    const double *__restrict__ cx;    // [3][NT]
+    const double *__restrict__ fn;    // [3][F]
+    const double *__restrict__ fa;    // [F]
+    const double *__restrict__ beta;  // [F]
+    const int *__restrict__ f0;       // [F]
+    const int *__restrict__ f1;       // [F]
+    const int *__restrict__ voff;     // [N+1]
+    const int *__restrict__ vface;    // [nnz]
+    const int *__restrict__ loff;     // [N+1]
+    const int *__restrict__ lface;    // [nnz]
+    const int *__restrict__ lnb;      // [nnz] neighbour across lface
+    const double *__restrict__ lsqc;  // [3][nnz] least-squares weights w*theta_k per list entry
+    const double *__restrict__ S;     // [3][N] projected areas (time step mode 2)
+    const int *__restrict__ gface;    // [G]
+    const int *__restrict__ gnb;      // [G]
+    const int *__restrict__ gbc;      // [G]
+};
+
+struct BCDev {
+    int dyn, th;
+    double u[3], p, T;
+};
+
+struct Phys {
+    double gamma, rm, cp, cv, cfl, dt_uniform, venkat_k;
+};
+
+template <typename T>
+struct State {
+    const T *__restrict__ q;  // [(dim+3)][NT]  stage-start primitives
+    T *__restrict__ qn;       // [(dim+3)][NT]  new primitives
+    T *__restrict__ W0;       // [(dim+2)][N]   conservative state at iteration start
+    T *__restrict__ grad;     // [(dim+2)][dim][N]
+    T *__restrict__ FA;       // [(dim+2)][F]
+};
+
+// ---------------------------------------------------------------------------
+// Small device helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T dsqrt(T x) {
+    return sqrt(x);
+}
+
+// psstScalarLimiters.cpp:17-93; eps is absolute (physical units).
+template <typename T, int LIM>
+__device__ __forceinline__ T limiter(T num, T den) {
+    const T eps = T(1.0e-10);
+    if ((num > T(0) && den < T(0)) || (num < T(0) && den > T(0))) return T(0);
+    if (LIM == PSST_LIMITER_VAN_ALBADA) {
+        if (fabs(den) < eps) return T(1);
+        T r = num / den;
+        return (r * r + r) / (r * r + T(1));
+    } else if (LIM == PSST_LIMITER_MINMOD) {
+        if (fabs(den) < eps) return T(1);
+        T r = num / den;
+        T res = r;
+        if (r > T(1)) res = T(1);
+        return res;
+    } else {  // Van Leer
+        if (fabs(den) < eps) return T(2);
+        T r = num / den;
+        return T(2) * r / (T(1) + r);
+    }
+}
+
+// psstConvectiveFlow.cpp:13-24
+template <typename T>
+__device__ __forceinline__ T roe_avg(T lv, T rv, T ld, T rd) {
+    T l = dsqrt(ld);
+    T r = dsqrt(rd);
+    T num = lv * l + rv * r;
+    T den = (l + r);
+    return num / den;
+}
+
+// ---------------------------------------------------------------------------
+// Gradients
+// ---------------------------------------------------------------------------
+// psstGreenGauss::calculate (psstGradCalculation.cpp:80-177), per interior
+// cell: sum over the cell's faces in face-loop order of
+// ((+-phi_f) * (A/V)) * n_k with phi_f = beta*phi_0 + (1-beta)*phi_1.
+template <int DIM, typename T>
+__global__ void __launch_bounds__(kBlock) k_grad_gg(Geom g, State<T> s) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    constexpr int NV = DIM + 2;
+    T acc[NV][DIM];
+#pragma unroll
+    for (int n = 0; n < NV; ++n)
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) acc[n][k] = T(0);
+    const double Vc = g.vol[c];
+    const int b = g.voff[c], e = g.voff[c + 1];
+    for (int i = b; i < e; ++i) {
+        const int f = g.vface[i];
+        const int a0 = g.f0[f], a1 = g.f1[f];
+        const T geom = T(g.fa[f] / Vc);
+        const T beta = T(g.beta[f]);
+        const T rbeta = T(1) - beta;
+        T nrm[DIM];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) nrm[k] = T(g.fn[(size_t)k * g.F + f]);
+        const bool first = (a0 == c);
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+            T phi = beta * s.q[(size_t)n * g.NT + a0] + rbeta * s.q[(size_t)n * g.NT + a1];
+            T sp = first ? -phi : phi;
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) acc[n][k] += sp * geom * nrm[k];
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < NV; ++n)
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) s.grad[((size_t)n * DIM + k) * g.N + c] = acc[n][k];
+}
+
+// psstLeastSquare::calculate (psstGradCalculation.cpp:268-368), per interior
+// cell over its face list; w*theta_k is precomputed on the host with the
+// reference's operation order, so each term is (w*theta_k) * (phi_nb - phi_c).
+template <int DIM, typename T>
+__global__ void __launch_bounds__(kBlock) k_grad_lsq(Geom g, State<T> s) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    constexpr int NV = DIM + 2;
+    T acc[NV][DIM];
+    T qc[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+        qc[n] = s.q[(size_t)n * g.NT + c];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) acc[n][k] = T(0);
+    }
+    const int b = g.loff[c], e = g.loff[c + 1];
+    const int nnz = g.loff[g.N];
+    for (int i = b; i < e; ++i) {
+        const int nb = g.lnb[i];
+        T w[DIM];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) w[k] = T(g.lsqc[(size_t)k * nnz + i]);
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+            T d = (s.q[(size_t)n * g.NT + nb] - qc[n]);
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) acc[n][k] += w[k] * d;
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < NV; ++n)
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) s.grad[((size_t)n * DIM + k) * g.N + c] = acc[n][k];
+}
+
+// ---------------------------------------------------------------------------
+// Face fluxes
+// ---------------------------------------------------------------------------
+template <int DIM, typename T>
+struct Side {
+    T u[DIM], p, T_, rho;
+};
+
+// psstReconstructorDM::reconstruct_vals (psstValueReconstructor.cpp:70-134)
+template <int DIM, typename T, int LIM>
+__device__ __forceinline__ void reconstruct_dm(const Geom &g, const State<T> &s, int a0, int a1, Side<DIM, T> &r0,
+                                               Side<DIM, T> &r1, double rm) {
+    constexpr int NV = DIM + 2;
+    T RLR[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) RLR[k] = T(g.cx[(size_t)k * g.NT + a0] - g.cx[(size_t)k * g.NT + a1]);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        T *p0 = (i < DIM) ? &r0.u[i] : (i == DIM ? &r0.p : &r0.T_);
+        T *p1 = (i < DIM) ? &r1.u[i] : (i == DIM ? &r1.p : &r1.T_);
+        T dl = T(0), dr = T(0);
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            dl += s.grad[((size_t)i * DIM + k) * g.N + a1] * RLR[k];
+            dr += s.grad[((size_t)i * DIM + k) * g.N + a0] * RLR[k];
+        }
+        const T v0 = *p0, v1 = *p1;
+        T vlm = v1 - (T(2) * dl - (v0 - v1));
+        T vlp = v0;
+        T vrm = v1;
+        T vrp = v0 + (T(2) * dr - (v0 - v1));
+        T nl = vlp - v1;
+        T dnl = v1 - vlm;
+        T nr = vrm - v0;
+        T dnr = v0 - vrp;
+        *p0 = v0 + T(0.5) * limiter<T, LIM>(nr, dnr) * dnr;
+        *p1 = v1 + T(0.5) * limiter<T, LIM>(nl, dnl) * dnl;
+    }
+    r0.rho = r0.p / (T(rm) * r0.T_);
+    r1.rho = r1.p / (T(rm) * r1.T_);
+}
+
+template <int DIM, typename T>
+__device__ __forceinline__ void load_side(const State<T> &s, int NT, int a, Side<DIM, T> &r) {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) r.u[k] = s.q[(size_t)k * NT + a];
+    r.p = s.q[(size_t)DIM * NT + a];
+    r.T_ = s.q[(size_t)(DIM + 1) * NT + a];
+    r.rho = s.q[(size_t)(DIM + 2) * NT + a];
+}
+
+// One face: FA[k][f] = F_k * A, the flux from neighbour 1 into neighbour 0
+// (R_0 -= FA, R_1 += FA).
+template <int DIM, typename T, int FLUX, int ORDER, int LIM>
+__global__ void __launch_bounds__(kBlock) k_flux(Geom g, State<T> s, Phys ph) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= g.F) return;
+    constexpr int WS = DIM + 2;
+    const int a0 = g.f0[f], a1 = g.f1[f];
+    T nrm[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) nrm[k] = T(g.fn[(size_t)k * g.F + f]);
+    const T A = T(g.fa[f]);
+    const T cp = T(ph.cp), cv = T(ph.cv), rm = T(ph.rm), gam = T(ph.gamma);
+    T flow[WS];
+    const bool g0 = a0 >= g.N, g1 = a1 >= g.N;
+    if (g0 || g1) {
+        // boundary face: physical flux of the ghost state (quirk Q5),
+        // psstConvectiveHLL::calculate :139-201 (same in HLLC/AUSM)
+        Side<DIM, T> gs;
+        load_side<DIM, T>(s, g.NT, g0 ? a0 : a1, gs);
+        T vn = T(0), vsq = T(0);
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            vn += gs.u[k] * nrm[k];
+            vsq += gs.u[k] * gs.u[k];
+        }
+        T H = cp * gs.T_ + T(0.5) * vsq;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) flow[k] = gs.rho * gs.u[k] * vn + gs.p * nrm[k];
+        flow[DIM] = gs.rho * vn;
+        flow[DIM + 1] = gs.rho * vn * H;
+    } else {
+        Side<DIM, T> r[2];
+        load_side<DIM, T>(s, g.NT, a0, r[0]);
+        load_side<DIM, T>(s, g.NT, a1, r[1]);
+        if (ORDER == 2) reconstruct_dm<DIM, T, LIM>(g, s, a0, a1, r[0], r[1], ph.rm);
+        T vn[2], a[2], H[2], E[2], fi[2][WS], wi[2][WS], M[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            vn[j] = T(0);
+            T vsq = T(0);
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) {
+                vn[j] += r[j].u[k] * nrm[k];
+                vsq += r[j].u[k] * r[j].u[k];
+            }
+            a[j] = dsqrt(gam * rm * r[j].T_);
+            H[j] = cp * r[j].T_ + T(0.5) * vsq;
+            if (FLUX == PSST_FLUX_AUSM) {
+                M[j] = vn[j] / a[j];
+            } else {
+                E[j] = cv * r[j].T_ + T(0.5) * vsq;
+#pragma unroll
+                for (int k = 0; k < DIM; ++k) {
+                    fi[j][k] = r[j].rho * r[j].u[k] * vn[j] + r[j].p * nrm[k];
+                    wi[j][k] = r[j].rho * r[j].u[k];
+                }
+                wi[j][DIM] = r[j].rho;
+                wi[j][DIM + 1] = r[j].rho * E[j];
+                fi[j][DIM] = r[j].rho * vn[j];
+                fi[j][DIM + 1] = r[j].rho * vn[j] * H[j];
+            }
+        }
+        if (FLUX == PSST_FLUX_HLL || FLUX == PSST_FLUX_HLLC) {
+            // psstConvectiveHLL::calculate :247-286, psstConvectiveHLLC::calculate :531-585
+            T Hr = roe_avg(H[1], H[0], r[1].rho, r[0].rho);
+            T vrn = T(0), vsq = T(0);
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) {
+                T vr = roe_avg(r[1].u[k], r[0].u[k], r[1].rho, r[0].rho);
+                vrn += vr * nrm[k];
+                vsq += vr * vr;
+            }
+            T ar = dsqrt((gam - T(1)) * (Hr - T(0.5) * vsq));
+            T s1 = vn[1] - a[1];
+            T tmp = vrn - ar;
+            if (s1 > tmp) s1 = tmp;
+            T s0 = vn[0] + a[0];
+            tmp = vrn + ar;
+            if (s0 < tmp) s0 = tmp;
+            if (FLUX == PSST_FLUX_HLL) {
+                if (s1 >= T(0)) {
+#pragma unroll
+                    for (int j = 0; j < WS; ++j) flow[j] = fi[1][j];
+                } else if (s1 <= T(0) && s0 >= T(0)) {
+#pragma unroll
+                    for (int j = 0; j < WS; ++j)
+                        flow[j] = (s0 * fi[1][j] - s1 * fi[0][j] + s0 * s1 * (wi[0][j] - wi[1][j])) / (s0 - s1);
+                } else {
+                    // s0 <= 0, or NaN speeds (the reference throws, quirk Q13;
+                    // the NaN propagates into the state and is reported)
+#pragma unroll
+                    for (int j = 0; j < WS; ++j) flow[j] = fi[0][j];
+                }
+            } else {
+                T sh = (r[0].p - r[1].p + r[1].rho * vn[1] * (s1 - vn[1]) - r[0].rho * vn[0] * (s0 - vn[0])) /
+                       (r[1].rho * (s1 - vn[1]) - r[0].rho * (s0 - vn[0]));
+                if (s1 >= T(0)) {
+#pragma unroll
+                    for (int j = 0; j < WS; ++j) flow[j] = fi[1][j];
+                } else if ((s1 <= T(0) && sh >= T(0)) || (sh <= T(0) && s0 >= T(0))) {
+                    const int K = (s1 <= T(0) && sh >= T(0)) ? 1 : 0;
+                    const T sK = K ? s1 : s0;
+                    const T vK = vn[K];
+                    const T tt = sK - sh;
+                    const T PK = r[K].p + r[K].rho * (sK - vK) * (sh - vK);
+#pragma unroll
+                    for (int j = 0; j < DIM; ++j) flow[j] = (sh * (sK * wi[K][j] - fi[K][j]) + sK * PK * nrm[j]) / tt;
+                    flow[DIM] = (sh * (sK * wi[K][DIM] - fi[K][DIM])) / tt;
+                    flow[DIM + 1] = (sh * (sK * wi[K][DIM + 1] - fi[K][DIM + 1]) + sK * PK * sh) / tt;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < WS; ++j) flow[j] = fi[0][j];
+                }
+            }
+        } else {
+            // psstConvectiveAUSM::calculate :804-851
+            T Ms1, ps1, Ms0, ps0;
+            if (M[1] >= T(1)) {
+                Ms1 = M[1];
+                ps1 = r[1].p;
+            } else if (M[1] <= T(-1)) {
+                Ms1 = T(0);
+                ps1 = T(0);
+            } else {
+                Ms1 = T(0.25) * (M[1] + T(1)) * (M[1] + T(1));
+                ps1 = T(0.25) * r[1].p * (M[1] + T(1)) * (M[1] + T(1)) * (T(2) - M[1]);
+            }
+            if (M[0] >= T(1)) {
+                Ms0 = T(0);
+                ps0 = T(0);
+            } else if (M[0] <= T(-1)) {
+                Ms0 = M[0];
+                ps0 = r[0].p;
+            } else {
+                Ms0 = T(-0.25) * (M[0] - T(1)) * (M[0] - T(1));
+                ps0 = T(0.25) * r[0].p * (M[0] - T(1)) * (M[0] - T(1)) * (T(2) + M[0]);
+            }
+            T Mf = Ms0 + Ms1;
+            T pf = ps0 + ps1;
+            const int K = (Mf > T(0)) ? 1 : 0;
+            T tmp = Mf * a[K] * r[K].rho;
+#pragma unroll
+            for (int j = 0; j < DIM; ++j) {
+                T pflow = pf * nrm[j];
+                flow[j] = tmp * r[K].u[j] + pflow;
+            }
+            flow[DIM] = tmp;
+            flow[DIM + 1] = tmp * H[K];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < WS; ++k) s.FA[(size_t)k * g.F + f] = flow[k] * A;
+}
+
+// ---------------------------------------------------------------------------
+// Time step, update, primitives
+// ---------------------------------------------------------------------------
+// psstLocalAccTime1::calculate_at (psstLocalTimeCalculation.cpp:90-131).  The
+// running normal-velocity average is not reset between faces (quirk Q3).
+// Neighbour states are read through `nbq`, which decides between the new and
+// the stage-start buffer.
+template <int DIM, typename T, typename NbQ>
+__device__ __forceinline__ T dt_local1(const Geom &g, int c, const T *uc, T Tc, const Phys &ph, NbQ nbq) {
+    const T gr = T(ph.gamma) * T(ph.rm);
+    T cc = dsqrt(gr * Tc);
+    T den = T(0), nn = T(0);
+    const int b = g.loff[c], e = g.loff[c + 1];
+    for (int i = b; i < e; ++i) {
+        const int f = g.lface[i];
+        const int nb = g.lnb[i];
+        T nv = T(0);
+        T Tn, un[DIM];
+        nbq(nb, un, Tn);
+        T cn = (dsqrt(gr * Tn) + cc) * T(0.5);
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            T t = T(g.fn[(size_t)k * g.F + f]);
+            nv += uc[k] * t;
+            nn += un[k] * t;
+        }
+        nn = (nn + nv) * T(0.5);
+        den += (fabs(nn) + cn) * T(g.fa[f]);
+    }
+    return T(ph.cfl) * T(g.vol[c]) / den;
+}
+
+// psstLocalAccTime2::calculate_at (psstLocalTimeCalculation.cpp:175-213);
+// S_k = sum_f A|n_k| is precomputed on the host in face-list order.
+template <int DIM, typename T>
+__device__ __forceinline__ T dt_local2(const Geom &g, int c, const T *uc, T Tc, const Phys &ph) {
+    T cc = dsqrt(T(ph.gamma) * T(ph.rm) * Tc);
+    T den = T(0);
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) den += (fabs(uc[j]) + cc) * T(g.S[(size_t)j * g.N + c]) * T(0.5);
+    return T(ph.cfl) * T(g.vol[c]) / den;
+}
+
+template <typename T>
+__device__ __forceinline__ unsigned long long abs_bits(T x) {
+    double d = fabs((double)x);
+    return (unsigned long long)__double_as_longlong(d);
+}
+
+// Block-wide max of the |R_j| bit patterns, then one integer atomic per block
+// and column into one of kResSlots slots.
+template <int WS>
+__device__ __forceinline__ void residual_max(unsigned long long (&m)[WS], unsigned long long *res_row) {
+    __shared__ unsigned long long sm[kBlock / 64][WS];
+#pragma unroll
+    for (int j = 0; j < WS; ++j) {
+        unsigned long long v = m[j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            unsigned long long o = __shfl_xor(v, off, 64);
+            v = v > o ? v : o;
+        }
+        m[j] = v;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int j = 0; j < WS; ++j) sm[wave][j] = m[j];
+    __syncthreads();
+    if (threadIdx.x < WS) {
+        unsigned long long v = 0;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) v = v > sm[w][threadIdx.x] ? v : sm[w][threadIdx.x];
+        if (v) atomicMax(&res_row[threadIdx.x * kResSlots + (blockIdx.x % kResSlots)], v);
+    }
+}
+
+// The body of the reference's update loop for one cell
+// (psstFiniteVolumeGrid.cpp:317-347).
+template <int DIM, typename T, int LC, typename NbQ>
+__device__ __forceinline__ void update_cell(const Geom &g, const State<T> &s, const Phys &ph, int c, T alpha,
+                                            bool first_stage, bool last_stage, unsigned long long (&rmax)[DIM + 2],
+                                            NbQ nbq) {
+    constexpr int WS = DIM + 2;
+    // residual: ordered gather of the face contributions (face-loop order)
+    T R[WS];
+#pragma unroll
+    for (int j = 0; j < WS; ++j) R[j] = T(0);
+    const int b = g.voff[c], e = g.voff[c + 1];
+    for (int i = b; i < e; ++i) {
+        const int f = g.vface[i];
+        const bool first = (g.f0[f] == c);
+#pragma unroll
+        for (int j = 0; j < WS; ++j) {
+            T v = s.FA[(size_t)j * g.F + f];
+            R[j] = first ? R[j] - v : R[j] + v;
+        }
+    }
+    T uc[DIM];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) uc[k] = s.q[(size_t)k * g.NT + c];
+    const T Tc = s.q[(size_t)(DIM + 1) * g.NT + c];
+    T dt;
+    if (LC == PSST_DT_UNIFORM) dt = T(ph.dt_uniform);
+    else if (LC == PSST_DT_LOCAL1) dt = dt_local1<DIM, T>(g, c, uc, Tc, ph, nbq);
+    else dt = dt_local2<DIM, T>(g, c, uc, Tc, ph);
+    const T V = T(g.vol[c]);
+    T w[WS];
+#pragma unroll
+    for (int j = 0; j < WS; ++j) {
+        T w0 = s.W0[(size_t)j * g.N + c];
+        w[j] = w0 - dt * alpha * R[j] / V;
+        unsigned long long ab = abs_bits(R[j]);
+        rmax[j] = rmax[j] > ab ? rmax[j] : ab;
+    }
+    (void)first_stage;
+    if (last_stage)
+#pragma unroll
+        for (int j = 0; j < WS; ++j) s.W0[(size_t)j * g.N + c] = w[j];
+    const T rho = w[DIM];
+    T vsq = T(0);
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+        T u = w[k] / rho;
+        s.qn[(size_t)k * g.NT + c] = u;
+        vsq += u * u;
+    }
+    T Tn = ((w[DIM + 1] / rho) - (vsq * T(0.5))) / T(ph.cv);
+    s.qn[(size_t)DIM * g.NT + c] = rho * T(ph.rm) * Tn;
+    s.qn[(size_t)(DIM + 1) * g.NT + c] = Tn;
+    s.qn[(size_t)(DIM + 2) * g.NT + c] = rho;
+}
+
+template <int DIM, typename T>
+struct NbOld {
+    const T *q;
+    int NT;
+    __device__ __forceinline__ void operator()(int nb, T *un, T &Tn) const {
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) un[k] = q[(size_t)k * NT + nb];
+        Tn = q[(size_t)(DIM + 1) * NT + nb];
+    }
+};
+
+// Reference order: an interior neighbour earlier in the update loop has its
+// new state already (psstFiniteVolumeGrid.cpp:317-347 is an in-place sweep).
+template <int DIM, typename T>
+struct NbOrdered {
+    const T *q;
+    const T *qn;
+    int NT;
+    int c;
+    __device__ __forceinline__ void operator()(int nb, T *un, T &Tn) const {
+        const T *src = (nb < c) ? qn : q;  // canonical index == update order; ghosts are >= N > c
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) un[k] = src[(size_t)k * NT + nb];
+        Tn = src[(size_t)(DIM + 1) * NT + nb];
+    }
+};
+
+template <int DIM, typename T, int LC>
+__global__ void __launch_bounds__(kBlock) k_update(Geom g, State<T> s, Phys ph, T alpha, int first_stage,
+                                                   int last_stage, unsigned long long *res_row) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long rmax[DIM + 2];
+#pragma unroll
+    for (int j = 0; j < DIM + 2; ++j) rmax[j] = 0;
+    if (c < g.N) update_cell<DIM, T, LC>(g, s, ph, c, alpha, first_stage, last_stage, rmax, NbOld<DIM, T>{s.q, g.NT});
+    residual_max<DIM + 2>(rmax, res_row);
+}
+
+// Reference-ordered update for time step mode 1: the cells of one level
+// depend only on cells of lower levels.  One launch per level.
+template <int DIM, typename T>
+__global__ void __launch_bounds__(kBlock) k_update_level(Geom g, State<T> s, Phys ph, T alpha, int first_stage,
+                                                         int last_stage, unsigned long long *res_row,
+                                                         const int *__restrict__ cells, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long rmax[DIM + 2];
+#pragma unroll
+    for (int j = 0; j < DIM + 2; ++j) rmax[j] = 0;
+    if (i < count) {
+        const int c = cells[i];
+        update_cell<DIM, T, PSST_DT_LOCAL1>(g, s, ph, c, alpha, first_stage, last_stage, rmax,
+                                            NbOrdered<DIM, T>{s.q, s.qn, g.NT, c});
+    }
+    residual_max<DIM + 2>(rmax, res_row);
+}
+
+// Same, all levels in one workgroup with a barrier between levels (small
+// meshes, where launch latency would dominate).
+template <int DIM, typename T>
+__global__ void __launch_bounds__(1024) k_update_levels_block(Geom g, State<T> s, Phys ph, T alpha, int first_stage,
+                                                              int last_stage, unsigned long long *res_row,
+                                                              const int *__restrict__ cells,
+                                                              const int *__restrict__ level_off, int n_levels) {
+    unsigned long long rmax[DIM + 2];
+#pragma unroll
+    for (int j = 0; j < DIM + 2; ++j) rmax[j] = 0;
+    for (int l = 0; l < n_levels; ++l) {
+        const int b = level_off[l], e = level_off[l + 1];
+        for (int i = b + (int)threadIdx.x; i < e; i += (int)blockDim.x) {
+            const int c = cells[i];
+            update_cell<DIM, T, PSST_DT_LOCAL1>(g, s, ph, c, alpha, first_stage, last_stage, rmax,
+                                                NbOrdered<DIM, T>{s.q, s.qn, g.NT, c});
+        }
+        __syncthreads();
+    }
+    // block-wide max over 16 waves
+    __shared__ unsigned long long sm[16][DIM + 2];
+#pragma unroll
+    for (int j = 0; j < DIM + 2; ++j) {
+        unsigned long long v = rmax[j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            unsigned long long o = __shfl_xor(v, off, 64);
+            v = v > o ? v : o;
+        }
+        rmax[j] = v;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int j = 0; j < DIM + 2; ++j) sm[wave][j] = rmax[j];
+    __syncthreads();
+    if (threadIdx.x < DIM + 2) {
+        unsigned long long v = 0;
+        for (int w = 0; w < (int)blockDim.x / 64; ++w) v = v > sm[w][threadIdx.x] ? v : sm[w][threadIdx.x];
+        if (v) atomicMax(&res_row[threadIdx.x * kResSlots], v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary conditions (psstBoundaryCondition.cpp; ghost density
+// psstFiniteVolumeGrid.cpp:364).  Writes the ghost part of `qn` from the
+// interior part of `qn`.
+// ---------------------------------------------------------------------------
+template <int DIM, typename T>
+__global__ void __launch_bounds__(kBlock) k_bc(Geom g, T *__restrict__ qn, const BCDev *__restrict__ bcs, Phys ph) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.G) return;
+    const int gc = g.N + i;
+    const int c = g.gnb[i];
+    const BCDev bc = bcs[g.gbc[i]];
+    const size_t NT = (size_t)g.NT;
+    T u[DIM], p, Tt;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) u[k] = qn[k * NT + gc];
+    p = qn[DIM * NT + gc];
+    Tt = qn[(DIM + 1) * NT + gc];
+    switch (bc.dyn) {
+    case PSST_BC_SS_OUTLET:  // :72-80
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) u[k] = qn[k * NT + c];
+        p = qn[DIM * NT + c];
+        break;
+    case PSST_BC_SLIP_WALL: {  // :108-127
+        const int f = g.gface[i];
+        T vn = T(0);
+        T nrm[DIM];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            nrm[k] = T(g.fn[(size_t)k * g.F + f]);
+            vn += qn[k * NT + c] * nrm[k];
+        }
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) u[k] = qn[k * NT + c] - vn * nrm[k];
+        p = qn[DIM * NT + c];
+        break;
+    }
+    case PSST_BC_OUTLET:  // :212-219
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) u[k] = qn[k * NT + c];
+        break;
+    default:  // supersonic inlet: fixed state
+        break;
+    }
+    if (bc.th == PSST_BCT_OUTLET || bc.th == PSST_BCT_WALL_Q) Tt = qn[(DIM + 1) * NT + c];  // :285-288, :354-357
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) qn[k * NT + gc] = u[k];
+    qn[DIM * NT + gc] = p;
+    qn[(DIM + 1) * NT + gc] = Tt;
+    qn[(DIM + 2) * NT + gc] = p / (Tt * T(ph.rm));
+}
+
+// Conservative variables from primitives for the interior cells
+// (psstFiniteVolumeGrid.cpp:279-300).
+template <int DIM, typename T>
+__global__ void __launch_bounds__(kBlock) k_set_w(Geom g, const T *__restrict__ q, T *__restrict__ W0, Phys ph) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const size_t NT = (size_t)g.NT;
+    T rho = q[(DIM + 2) * NT + c];
+    T vsq = T(0);
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) {
+        T t = q[j * NT + c];
+        W0[(size_t)j * g.N + c] = rho * t;
+        vsq += t * t;
+    }
+    T E = T(ph.cv) * q[(DIM + 1) * NT + c] + T(0.5) * vsq;
+    W0[(size_t)DIM * g.N + c] = rho;
+    W0[(size_t)(DIM + 1) * g.N + c] = rho * E;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, size_t n, unsigned long long v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Gather-only pass (no update): R for diagnostics.
+template <int DIM, typename T>
+__global__ void __launch_bounds__(kBlock) k_gather_residual(Geom g, const T *__restrict__ FA, double *__restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    constexpr int WS = DIM + 2;
+    T R[WS];
+#pragma unroll
+    for (int j = 0; j < WS; ++j) R[j] = T(0);
+    for (int i = g.voff[c]; i < g.voff[c + 1]; ++i) {
+        const int f = g.vface[i];
+        const bool first = (g.f0[f] == c);
+#pragma unroll
+        for (int j = 0; j < WS; ++j) {
+            T v = FA[(size_t)j * g.F + f];
+            R[j] = first ? R[j] - v : R[j] + v;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < WS; ++j) out[(size_t)j * g.N + c] = (double)R[j];
+}
+
+template <typename A, typename B>
+__global__ void k_convert(const A *__restrict__ a, B *__restrict__ b, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (B)a[i];
+}
+
+}  // namespace
+
+// ===========================================================================
+// Context
+// ===========================================================================
+struct psst_ctx {
+    int dim = 2, N = 0, G = 0, F = 0, NT = 0, nnz_visit = 0, nnz_list = 0;
+    psst_settings s{};
+    std::vector<psst_bc> bc_host;
+    Phys ph{};
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    Geom geom{};
+    BCDev *bc = nullptr;
+    // state (double or float)
+    void *q[2] = {nullptr, nullptr};
+    void *W0 = nullptr, *grad = nullptr, *FA = nullptr;
+    int cur = 0;
+    bool have_state = false;
+    unsigned long long *res = nullptr;  // [cap][dim+2][kResSlots]
+    int res_cap = 0;
+    // reference-ordered time step schedule
+    int *level_cells = nullptr, *level_off_dev = nullptr;
+    std::vector<int> level_off;
+    int max_level = 0;
+    // residual normalisation for the tolerance test
+    std::vector<double> res0;
+    // timing
+    bool timing = false;
+    struct KStat {
+        int64_t count = 0;
+        double total_ms = 0.0;
+    };
+    std::map<std::string, KStat> stats;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<std::string> kernel_names;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename X>
+X *dev_alloc(psst_ctx *c, size_t n) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    hip_check(hipMalloc(&p, n * sizeof(X)), "hipMalloc");
+    c->allocs.push_back(p);
+    return static_cast<X *>(p);
+}
+
+template <typename X>
+X *dev_upload(psst_ctx *c, const X *h, size_t n) {
+    X *d = dev_alloc<X>(c, n);
+    if (n) hip_check(hipMemcpy(d, h, n * sizeof(X), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    return d;
+}
+
+hipEvent_t get_event(psst_ctx *c) {
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+
+void flush_timing(psst_ctx *c) {
+    if (c->pending.empty()) return;
+    hip_check(hipEventSynchronize(c->pending.back().second.second), "hipEventSynchronize");
+    for (auto &p : c->pending) {
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, p.second.first, p.second.second), "hipEventElapsedTime");
+        auto &st = c->stats[p.first];
+        st.count++;
+        st.total_ms += ms;
+        c->event_pool.push_back(p.second.first);
+        c->event_pool.push_back(p.second.second);
+    }
+    c->pending.clear();
+}
+
+template <typename K, typename... Args>
+void launch(psst_ctx *c, const char *name, K kernel, dim3 grid, dim3 block, Args... args) {
+    if (grid.x == 0) return;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        if (c->pending.size() > 4096) flush_timing(c);
+        e0 = get_event(c);
+        e1 = get_event(c);
+        hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
+    }
+    hipLaunchKernelGGL(kernel, grid, block, 0, c->stream, args...);
+    hip_check(hipGetLastError(), name);
+    if (c->timing) {
+        hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
+        c->pending.push_back({name, {e0, e1}});
+    }
+}
+
+inline unsigned blocks(long n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+
+template <typename T>
+State<T> state(psst_ctx *c) {
+    State<T> st;
+    st.q = static_cast<const T *>(c->q[c->cur]);
+    st.qn = static_cast<T *>(c->q[c->cur ^ 1]);
+    st.W0 = static_cast<T *>(c->W0);
+    st.grad = static_cast<T *>(c->grad);
+    st.FA = static_cast<T *>(c->FA);
+    return st;
+}
+
+// ---- dispatch of the templated kernels ----
+template <int DIM, typename T>
+void run_grad(psst_ctx *c) {
+    State<T> st = state<T>(c);
+    if (c->s.grad == PSST_GRAD_GREEN_GAUSS)
+        launch(c, "grad_gg", k_grad_gg<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st);
+    else
+        launch(c, "grad_lsq", k_grad_lsq<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st);
+}
+
+template <int DIM, typename T, int FLUX, int ORDER>
+void run_flux_lim(psst_ctx *c) {
+    State<T> st = state<T>(c);
+    dim3 gr(blocks(c->F)), bl(kBlock);
+    if (ORDER == 1) {
+        launch(c, "flux", k_flux<DIM, T, FLUX, 1, PSST_LIMITER_VAN_LEER>, gr, bl, c->geom, st, c->ph);
+        return;
+    }
+    switch (c->s.limiter) {
+    case PSST_LIMITER_VAN_ALBADA:
+        launch(c, "flux", k_flux<DIM, T, FLUX, 2, PSST_LIMITER_VAN_ALBADA>, gr, bl, c->geom, st, c->ph);
+        break;
+    case PSST_LIMITER_MINMOD:
+        launch(c, "flux", k_flux<DIM, T, FLUX, 2, PSST_LIMITER_MINMOD>, gr, bl, c->geom, st, c->ph);
+        break;
+    default:
+        launch(c, "flux", k_flux<DIM, T, FLUX, 2, PSST_LIMITER_VAN_LEER>, gr, bl, c->geom, st, c->ph);
+        break;
+    }
+}
+
+template <int DIM, typename T>
+void run_flux(psst_ctx *c) {
+    const bool o2 = c->s.order == 2;
+    switch (c->s.flux) {
+    case PSST_FLUX_AUSM:
+        o2 ? run_flux_lim<DIM, T, PSST_FLUX_AUSM, 2>(c) : run_flux_lim<DIM, T, PSST_FLUX_AUSM, 1>(c);
+        break;
+    case PSST_FLUX_HLL:
+        o2 ? run_flux_lim<DIM, T, PSST_FLUX_HLL, 2>(c) : run_flux_lim<DIM, T, PSST_FLUX_HLL, 1>(c);
+        break;
+    default:
+        o2 ? run_flux_lim<DIM, T, PSST_FLUX_HLLC, 2>(c) : run_flux_lim<DIM, T, PSST_FLUX_HLLC, 1>(c);
+        break;
+    }
+}
+
+template <int DIM, typename T>
+void run_update(psst_ctx *c, int k, unsigned long long *res_row) {
+    State<T> st = state<T>(c);
+    const T alpha = T(c->s.rk_coeff[k]);
+    const int first = (k == 0), last = (k == c->s.rk_levels - 1);
+    if (c->s.time_mode == PSST_DT_LOCAL1 && c->s.dt_ordering == 0) {
+        const int nl = (int)c->level_off.size() - 1;
+        if (c->max_level <= 4096) {
+            launch(c, "update_levels_block", k_update_levels_block<DIM, T>, dim3(1), dim3(1024), c->geom, st, c->ph,
+                   alpha, first, last, res_row, (const int *)c->level_cells, (const int *)c->level_off_dev, nl);
+        } else {
+            for (int l = 0; l < nl; ++l) {
+                int b = c->level_off[l], n = c->level_off[l + 1] - b;
+                launch(c, "update_level", k_update_level<DIM, T>, dim3(blocks(n)), dim3(kBlock), c->geom, st, c->ph,
+                       alpha, first, last, res_row, (const int *)(c->level_cells + b), n);
+            }
+        }
+    } else if (c->s.time_mode == PSST_DT_LOCAL1) {
+        launch(c, "update", k_update<DIM, T, PSST_DT_LOCAL1>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st, c->ph,
+               alpha, first, last, res_row);
+    } else if (c->s.time_mode == PSST_DT_LOCAL2) {
+        launch(c, "update", k_update<DIM, T, PSST_DT_LOCAL2>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st, c->ph,
+               alpha, first, last, res_row);
+    } else {
+        launch(c, "update", k_update<DIM, T, PSST_DT_UNIFORM>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st, c->ph,
+               alpha, first, last, res_row);
+    }
+}
+
+template <int DIM, typename T>
+void run_bc(psst_ctx *c) {
+    T *qn = static_cast<T *>(c->q[c->cur ^ 1]);
+    launch(c, "bc", k_bc<DIM, T>, dim3(blocks(c->G)), dim3(kBlock), c->geom, qn, (const BCDev *)c->bc, c->ph);
+}
+
+template <int DIM, typename T>
+void run_iteration(psst_ctx *c, unsigned long long *res_row) {
+    if (c->s.order == 2) run_grad<DIM, T>(c);
+    for (int k = 0; k < c->s.rk_levels; ++k) {
+        run_flux<DIM, T>(c);
+        run_update<DIM, T>(c, k, res_row);
+        run_bc<DIM, T>(c);
+        c->cur ^= 1;
+    }
+}
+
+template <int DIM, typename T>
+void run_set_w(psst_ctx *c) {
+    launch(c, "set_w", k_set_w<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, (const T *)c->q[c->cur],
+           (T *)c->W0, c->ph);
+}
+
+template <typename T>
+void dispatch_iteration(psst_ctx *c, unsigned long long *res_row) {
+    if (c->dim == 2) run_iteration<2, T>(c, res_row);
+    else run_iteration<3, T>(c, res_row);
+}
+
+void ensure_res(psst_ctx *c, int n_iter) {
+    if (n_iter <= c->res_cap) return;
+    int cap = std::max(n_iter, 64);
+    c->res = dev_alloc<unsigned long long>(c, (size_t)cap * (c->dim + 2) * kResSlots);
+    c->res_cap = cap;
+}
+
+size_t state_bytes(const psst_ctx *c) { return c->s.precision == 32 ? sizeof(float) : sizeof(double); }
+
+psst_status set_error(psst_ctx *c, const std::exception &e, psst_status st) {
+    if (c) c->err = e.what();
+    else g_create_error = e.what();
+    return st;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char *psst_last_error(const psst_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, psst_ctx **out) {
+    if (!m || !s || !out) {
+        g_create_error = "null argument";
+        return PSST_BAD_ARG;
+    }
+    *out = nullptr;
+    auto c = std::make_unique<psst_ctx>();
+    try {
+        if (m->dim != 2 && m->dim != 3) throw std::invalid_argument("dim must be 2 or 3");
+        if (m->n_cells <= 0 || m->n_ghosts < 0 || m->n_faces <= 0) throw std::invalid_argument("empty mesh");
+        if (s->rk_levels < 1 || s->rk_levels > 8) throw std::invalid_argument("rk_levels must be 1..8");
+        if (s->flux < 1 || s->flux > 3) throw std::invalid_argument("flux must be 1 (AUSM), 2 (HLL) or 3 (HLLC)");
+        if (s->order != 1 && s->order != 2) throw std::invalid_argument("order must be 1 or 2");
+        if (s->order == 2 && (s->limiter < 1 || s->limiter > 3))
+            throw std::invalid_argument("limiter must be 1..3 for second order");
+        if (s->time_mode < 0 || s->time_mode > 2) throw std::invalid_argument("time_mode must be 0, 1 or 2");
+        if (s->precision != 64 && s->precision != 32) throw std::invalid_argument("precision must be 64 or 32");
+        if (s->order == 2 && s->grad == PSST_GRAD_LEAST_SQUARES && (!m->lsq_r || !m->lsq_beta))
+            throw std::invalid_argument("least squares needs lsq_r and lsq_beta");
+        if (s->n_bc <= 0 || !s->bc) throw std::invalid_argument("boundary table missing");
+        c->dim = m->dim;
+        c->N = m->n_cells;
+        c->G = m->n_ghosts;
+        c->F = m->n_faces;
+        c->NT = c->N + c->G;
+        c->s = *s;
+        c->bc_host.assign(s->bc, s->bc + s->n_bc);
+        c->s.bc = nullptr;
+        c->ph = Phys{s->gamma, s->rm, s->cp, s->cv, s->cfl, s->uniform_dt, s->venkat_k};
+        const int dim = c->dim, N = c->N, G = c->G, F = c->F, NT = c->NT;
+        // ---- host-side validation of the topology (every index in range) ----
+        const int32_t *f0 = m->face_cells, *f1 = m->face_cells + F;
+        for (int f = 0; f < F; ++f) {
+            if (f0[f] < 0 || f0[f] >= NT || f1[f] < 0 || f1[f] >= NT || f0[f] == f1[f])
+                throw std::invalid_argument("face_cells out of range");
+            if (f0[f] >= N && f1[f] >= N) throw std::invalid_argument("face between two ghosts");
+        }
+        if (m->visit_offsets[0] != 0 || m->list_offsets[0] != 0) throw std::invalid_argument("CSR offsets must start at 0");
+        for (int i = 0; i < N; ++i) {
+            if (m->visit_offsets[i + 1] < m->visit_offsets[i] || m->list_offsets[i + 1] < m->list_offsets[i])
+                throw std::invalid_argument("CSR offsets not monotone");
+        }
+        c->nnz_visit = m->visit_offsets[N];
+        c->nnz_list = m->list_offsets[N];
+        std::vector<int> lnb((size_t)c->nnz_list);
+        for (int i = 0; i < N; ++i) {
+            for (int k = m->visit_offsets[i]; k < m->visit_offsets[i + 1]; ++k) {
+                int f = m->visit_faces[k];
+                if (f < 0 || f >= F || (f0[f] != i && f1[f] != i)) throw std::invalid_argument("visit list corrupt");
+            }
+            for (int k = m->list_offsets[i]; k < m->list_offsets[i + 1]; ++k) {
+                int f = m->list_faces[k];
+                if (f < 0 || f >= F || (f0[f] != i && f1[f] != i)) throw std::invalid_argument("face list corrupt");
+                lnb[(size_t)k] = (f0[f] == i) ? f1[f] : f0[f];
+            }
+        }
+        std::vector<int> gnb((size_t)std::max(G, 1));
+        for (int i = 0; i < G; ++i) {
+            int f = m->ghost_face[i];
+            if (f < 0 || f >= F || (f0[f] != N + i && f1[f] != N + i)) throw std::invalid_argument("ghost_face corrupt");
+            if (m->ghost_bc[i] < 0 || m->ghost_bc[i] >= s->n_bc) throw std::invalid_argument("ghost_bc out of range");
+            gnb[(size_t)i] = (f0[f] == N + i) ? f1[f] : f0[f];
+            const psst_bc &b = s->bc[m->ghost_bc[i]];
+            int dk = b.dynamic_kind, tk = b.thermal_kind;
+            if (dk != PSST_BC_SS_INLET && dk != PSST_BC_SS_OUTLET && dk != PSST_BC_SLIP_WALL && dk != PSST_BC_OUTLET)
+                throw std::invalid_argument("unsupported dynamic boundary kind");
+            if (tk != PSST_BCT_WALL_T && tk != PSST_BCT_WALL_Q && tk != PSST_BCT_INLET && tk != PSST_BCT_OUTLET)
+                throw std::invalid_argument("unsupported thermal boundary kind");
+        }
+        // ---- derived host tables ----
+        // projected areas for time step mode 2, in face-list order
+        // (psstLocalTimeCalculation.cpp:192-202)
+        std::vector<double> S(3 * (size_t)N, 0.0);
+        for (int i = 0; i < N; ++i) {
+            double sk[3] = {0, 0, 0};
+            for (int k = m->list_offsets[i]; k < m->list_offsets[i + 1]; ++k) {
+                int f = m->list_faces[k];
+                for (int d = 0; d < dim; ++d) sk[d] += m->face_area[f] * std::abs(m->face_normal[(size_t)d * F + f]);
+            }
+            for (int d = 0; d < 3; ++d) S[(size_t)d * N + i] = sk[d];
+        }
+        // least-squares weights w*theta_k per list entry
+        // (psstGradCalculation.cpp:309-356; 3D: textbook QR weights)
+        std::vector<double> lsqc(3 * (size_t)std::max(c->nnz_list, 1), 0.0);
+        if (s->grad == PSST_GRAD_LEAST_SQUARES && m->lsq_r) {
+            for (int i = 0; i < N; ++i) {
+                const double ra = m->lsq_r[(size_t)0 * N + i], rb = m->lsq_r[(size_t)1 * N + i],
+                             rd = m->lsq_r[(size_t)2 * N + i], re = m->lsq_r[(size_t)4 * N + i],
+                             rg = m->lsq_r[(size_t)5 * N + i], beta = m->lsq_beta[i];
+                for (int k = m->list_offsets[i]; k < m->list_offsets[i + 1]; ++k) {
+                    int nb = lnb[(size_t)k];
+                    double diff[3] = {0, 0, 0}, w = 0.0, alpha[3] = {0, 0, 0}, theta[3] = {0, 0, 0};
+                    for (int d = 0; d < dim; ++d) {
+                        diff[d] = m->cell_centre[(size_t)d * NT + nb] - m->cell_centre[(size_t)d * NT + i];
+                        w += diff[d] * diff[d];
+                    }
+                    double ws = 1.0 / w;
+                    if (dim == 2) {
+                        alpha[0] = diff[0] / (ra * ra);
+                        alpha[1] = (diff[1] - rb / ra * diff[0]) / (rd * rd);
+                        theta[0] = alpha[0] - rb / ra * alpha[1];
+                        theta[1] = alpha[1];
+                    } else {
+                        alpha[0] = diff[0] / (ra * ra);
+                        alpha[1] = (diff[1] - rb / ra * diff[0]) / (rd * rd);
+                        alpha[2] = (diff[2] - re / rd * diff[1] + beta * diff[0]) / (rg * rg);
+                        theta[0] = alpha[0] - rb / ra * alpha[1] + beta * alpha[2];
+                        theta[1] = alpha[1] - re / rd * alpha[2];
+                        theta[2] = alpha[2];
+                    }
+                    for (int d = 0; d < 3; ++d) lsqc[(size_t)d * c->nnz_list + k] = ws * theta[d];
+                }
+            }
+        }
+        // reference-ordered schedule for time step mode 1: level of a cell =
+        // 1 + max level of its interior face-list neighbours earlier in the
+        // update loop (canonical index order)
+        if (s->time_mode == PSST_DT_LOCAL1 && s->dt_ordering == 0) {
+            std::vector<int> lev((size_t)N, 0);
+            int maxl = 0;
+            for (int i = 0; i < N; ++i) {
+                int l = 0;
+                for (int k = m->list_offsets[i]; k < m->list_offsets[i + 1]; ++k) {
+                    int nb = lnb[(size_t)k];
+                    if (nb < i) l = std::max(l, lev[(size_t)nb] + 1);
+                }
+                lev[(size_t)i] = l;
+                maxl = std::max(maxl, l);
+            }
+            c->level_off.assign((size_t)maxl + 2, 0);
+            for (int i = 0; i < N; ++i) c->level_off[(size_t)lev[(size_t)i] + 1]++;
+            for (int l = 0; l <= maxl; ++l) c->level_off[(size_t)l + 1] += c->level_off[(size_t)l];
+            std::vector<int> cells((size_t)N), fill(c->level_off.begin(), c->level_off.end() - 1);
+            int widest = 0;
+            for (int i = 0; i < N; ++i) cells[(size_t)fill[(size_t)lev[(size_t)i]]++] = i;
+            for (int l = 0; l <= maxl; ++l) widest = std::max(widest, c->level_off[(size_t)l + 1] - c->level_off[(size_t)l]);
+            c->max_level = widest;
+            hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
+            c->level_cells = dev_upload(c.get(), cells.data(), cells.size());
+            c->level_off_dev = dev_upload(c.get(), c->level_off.data(), c->level_off.size());
+        } else {
+            hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
+        }
+        // ---- device geometry ----
+        Geom &g = c->geom;
+        g.N = N;
+        g.G = G;
+        g.NT = NT;
+        g.F = F;
+        std::vector<double> cx3(3 * (size_t)NT, 0.0), fn3(3 * (size_t)F, 0.0);
+        for (int d = 0; d < dim; ++d) {
+            std::memcpy(&cx3[(size_t)d * NT], m->cell_centre + (size_t)d * NT, sizeof(double) * NT);
+            std::memcpy(&fn3[(size_t)d * F], m->face_normal + (size_t)d * F, sizeof(double) * F);
+        }
+        g.vol = dev_upload(c.get(), m->cell_volume, (size_t)NT);
+        g.cx = dev_upload(c.get(), cx3.data(), cx3.size());
+        g.fn = dev_upload(c.get(), fn3.data(), fn3.size());
+        g.fa = dev_upload(c.get(), m->face_area, (size_t)F);
+        g.beta = dev_upload(c.get(), m->face_beta, (size_t)F);
+        g.f0 = dev_upload(c.get(), f0, (size_t)F);
+        g.f1 = dev_upload(c.get(), f1, (size_t)F);
+        g.voff = dev_upload(c.get(), m->visit_offsets, (size_t)N + 1);
+        g.vface = dev_upload(c.get(), m->visit_faces, (size_t)c->nnz_visit);
+        g.loff = dev_upload(c.get(), m->list_offsets, (size_t)N + 1);
+        g.lface = dev_upload(c.get(), m->list_faces, (size_t)c->nnz_list);
+        g.lnb = dev_upload(c.get(), lnb.data(), lnb.size());
+        g.lsqc = dev_upload(c.get(), lsqc.data(), lsqc.size());
+        g.S = dev_upload(c.get(), S.data(), S.size());
+        g.gface = dev_upload(c.get(), m->ghost_face, (size_t)G);
+        g.gnb = dev_upload(c.get(), gnb.data(), (size_t)G);
+        g.gbc = dev_upload(c.get(), m->ghost_bc, (size_t)G);
+        std::vector<BCDev> bcd((size_t)s->n_bc);
+        for (int i = 0; i < s->n_bc; ++i) {
+            bcd[(size_t)i].dyn = s->bc[i].dynamic_kind;
+            bcd[(size_t)i].th = s->bc[i].thermal_kind;
+            for (int d = 0; d < 3; ++d) bcd[(size_t)i].u[d] = s->bc[i].velocity[d];
+            bcd[(size_t)i].p = s->bc[i].pressure;
+            bcd[(size_t)i].T = s->bc[i].temperature;
+        }
+        c->bc = dev_upload(c.get(), bcd.data(), bcd.size());
+        // ---- state ----
+        const size_t sb = state_bytes(c.get());
+        for (int i = 0; i < 2; ++i) {
+            hip_check(hipMalloc(&c->q[i], sb * (size_t)(dim + 3) * NT), "hipMalloc");
+            c->allocs.push_back(c->q[i]);
+        }
+        hip_check(hipMalloc(&c->W0, sb * (size_t)(dim + 2) * N), "hipMalloc");
+        c->allocs.push_back(c->W0);
+        hip_check(hipMalloc(&c->grad, sb * (size_t)(dim + 2) * dim * N), "hipMalloc");
+        c->allocs.push_back(c->grad);
+        hip_check(hipMalloc(&c->FA, sb * (size_t)(dim + 2) * F), "hipMalloc");
+        c->allocs.push_back(c->FA);
+        ensure_res(c.get(), 64);
+        c->kernel_names = {"grad_gg", "grad_lsq", "flux", "update", "update_level", "update_levels_block", "bc"};
+    } catch (const std::exception &e) {
+        g_create_error = e.what();
+        for (void *p : c->allocs) (void)hipFree(p);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        return dynamic_cast<const HipError *>(&e) ? PSST_HIP_ERROR : PSST_BAD_ARG;
+    }
+    *out = c.release();
+    return PSST_OK;
+}
+
+psst_status psst_set_state(psst_ctx *c, const double *prim) {
+    if (!c || !prim) return PSST_BAD_ARG;
+    try {
+        const size_t n = (size_t)(c->dim + 3) * c->NT;
+        if (c->s.precision == 64) {
+            hip_check(hipMemcpy(c->q[0], prim, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+            hip_check(hipMemcpy(c->q[1], prim, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        } else {
+            double *tmp = nullptr;
+            hip_check(hipMalloc(&tmp, n * sizeof(double)), "hipMalloc");
+            hip_check(hipMemcpy(tmp, prim, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks((long)n)), dim3(kBlock), 0, c->stream, tmp,
+                               (float *)c->q[0], n);
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks((long)n)), dim3(kBlock), 0, c->stream, tmp,
+                               (float *)c->q[1], n);
+            hip_check(hipStreamSynchronize(c->stream), "convert");
+            hip_check(hipFree(tmp), "hipFree");
+        }
+        c->cur = 0;
+        if (c->dim == 2) {
+            c->s.precision == 64 ? run_set_w<2, double>(c) : run_set_w<2, float>(c);
+        } else {
+            c->s.precision == 64 ? run_set_w<3, double>(c) : run_set_w<3, float>(c);
+        }
+        hip_check(hipStreamSynchronize(c->stream), "set_w");
+        c->have_state = true;
+        return PSST_OK;
+    } catch (const std::exception &e) {
+        return set_error(c, e, PSST_HIP_ERROR);
+    }
+}
+
+psst_status psst_set_residual_reference(psst_ctx *c, const double *res0) {
+    if (!c || !res0) return PSST_BAD_ARG;
+    c->res0.assign(res0, res0 + c->dim + 2);
+    return PSST_OK;
+}
+
+psst_status psst_run(psst_ctx *c, int32_t n_iter, double tol, double *res_hist, int32_t *iters_done) {
+    if (!c) return PSST_BAD_ARG;
+    if (!c->have_state) {
+        c->err = "psst_run: no state (call psst_set_state first)";
+        return PSST_BAD_ARG;
+    }
+    if (iters_done) *iters_done = 0;
+    if (n_iter <= 0) return PSST_OK;
+    const int ws = c->dim + 2;
+    try {
+        // residual rows are produced in chunks so a tolerance can stop the run
+        const int chunk = tol > 0.0 ? std::min<int32_t>(n_iter, 50) : n_iter;
+        ensure_res(c, chunk);
+        std::vector<unsigned long long> slots((size_t)chunk * ws * kResSlots);
+        int done = 0;
+        bool stop = false;
+        bool bad = false;
+        while (done < n_iter && !stop) {
+            const int n = std::min(chunk, n_iter - done);
+            const size_t nres = (size_t)n * ws * kResSlots;
+            launch(c, "fill", k_fill_u64, dim3(blocks((long)nres)), dim3(kBlock), c->res, nres, 0ull);
+            for (int it = 0; it < n; ++it) {
+                unsigned long long *row = c->res + (size_t)it * ws * kResSlots;
+                c->s.precision == 64 ? dispatch_iteration<double>(c, row) : dispatch_iteration<float>(c, row);
+            }
+            hip_check(hipMemcpyAsync(slots.data(), c->res, nres * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                     c->stream),
+                      "hipMemcpy D2H");
+            hip_check(hipStreamSynchronize(c->stream), "psst_run");
+            if (c->timing) flush_timing(c);
+            for (int it = 0; it < n; ++it) {
+                double row[5];
+                for (int j = 0; j < ws; ++j) {
+                    unsigned long long mx = 0;
+                    for (int sl = 0; sl < kResSlots; ++sl)
+                        mx = std::max(mx, slots[((size_t)it * ws + j) * kResSlots + sl]);
+                    double v;
+                    std::memcpy(&v, &mx, sizeof(v));
+                    row[j] = v;
+                    if (!(v <= 1.0e300)) bad = true;  // NaN or inf
+                }
+                if (res_hist)
+                    for (int j = 0; j < ws; ++j) res_hist[(size_t)(done + it) * ws + j] = row[j];
+                if (c->res0.empty()) c->res0.assign(row, row + ws);
+                if (tol > 0.0 && !stop) {
+                    double drop = 0.0;
+                    for (int j = 0; j < ws; ++j) drop = std::max(drop, c->res0[(size_t)j] > 0 ? row[j] / c->res0[(size_t)j] : 0.0);
+                    if (drop <= tol) {
+                        stop = true;
+                        if (iters_done) *iters_done = done + it + 1;
+                    }
+                }
+            }
+            done += n;
+            if (!stop && iters_done) *iters_done = done;
+        }
+        if (bad) {
+            c->err = "non-finite residual: the state contains NaN/inf (a negative temperature or density upstream)";
+            return PSST_NAN_DETECTED;
+        }
+        if (tol > 0.0 && !stop) return PSST_NO_CONVERGENCE;
+        return PSST_OK;
+    } catch (const std::exception &e) {
+        return set_error(c, e, PSST_HIP_ERROR);
+    }
+}
+
+psst_status psst_get_state(psst_ctx *c, double *prim) {
+    if (!c || !prim) return PSST_BAD_ARG;
+    try {
+        const size_t n = (size_t)(c->dim + 3) * c->NT;
+        if (c->s.precision == 64) {
+            hip_check(hipMemcpyAsync(prim, c->q[c->cur], n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+            hip_check(hipStreamSynchronize(c->stream), "D2H");
+        } else {
+            std::vector<float> tmp(n);
+            hip_check(hipMemcpyAsync(tmp.data(), c->q[c->cur], n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "D2H");
+            hip_check(hipStreamSynchronize(c->stream), "D2H");
+            for (size_t i = 0; i < n; ++i) prim[i] = tmp[i];
+        }
+        return PSST_OK;
+    } catch (const std::exception &e) {
+        return set_error(c, e, PSST_HIP_ERROR);
+    }
+}
+
+psst_status psst_get_conservative(psst_ctx *c, double *w) {
+    if (!c || !w) return PSST_BAD_ARG;
+    try {
+        const size_t n = (size_t)(c->dim + 2) * c->N;
+        if (c->s.precision == 64) {
+            hip_check(hipMemcpyAsync(w, c->W0, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+            hip_check(hipStreamSynchronize(c->stream), "D2H");
+        } else {
+            std::vector<float> tmp(n);
+            hip_check(hipMemcpyAsync(tmp.data(), c->W0, n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "D2H");
+            hip_check(hipStreamSynchronize(c->stream), "D2H");
+            for (size_t i = 0; i < n; ++i) w[i] = tmp[i];
+        }
+        return PSST_OK;
+    } catch (const std::exception &e) {
+        return set_error(c, e, PSST_HIP_ERROR);
+    }
+}
+
+psst_status psst_debug_gradients(psst_ctx *c, double *grads) {
+    if (!c || !grads) return PSST_BAD_ARG;
+    try {
+        if (c->dim == 2) {
+            c->s.precision == 64 ? run_grad<2, double>(c) : run_grad<2, float>(c);
+        } else {
+            c->s.precision == 64 ? run_grad<3, double>(c) : run_grad<3, float>(c);
+        }
+        const size_t n = (size_t)(c->dim + 2) * c->dim * c->N;
+        if (c->s.precision == 64) {
+            hip_check(hipMemcpyAsync(grads, c->grad, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+            hip_check(hipStreamSynchronize(c->stream), "D2H");
+        } else {
+            std::vector<float> tmp(n);
+            hip_check(hipMemcpyAsync(tmp.data(), c->grad, n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "D2H");
+            hip_check(hipStreamSynchronize(c->stream), "D2H");
+            for (size_t i = 0; i < n; ++i) grads[i] = tmp[i];
+        }
+        if (c->timing) flush_timing(c);
+        return PSST_OK;
+    } catch (const std::exception &e) {
+        return set_error(c, e, PSST_HIP_ERROR);
+    }
+}
+
+psst_status psst_debug_residual(psst_ctx *c, double *residual) {
+    if (!c || !residual) return PSST_BAD_ARG;
+    try {
+        double *out = nullptr;
+        const size_t n = (size_t)(c->dim + 2) * c->N;
+        hip_check(hipMalloc(&out, n * sizeof(double)), "hipMalloc");
+        if (c->dim == 2) {
+            if (c->s.precision == 64) {
+                run_flux<2, double>(c);
+                hipLaunchKernelGGL((k_gather_residual<2, double>), dim3(blocks(c->N)), dim3(kBlock), 0, c->stream,
+                                   c->geom, (const double *)c->FA, out);
+            } else {
+                run_flux<2, float>(c);
+                hipLaunchKernelGGL((k_gather_residual<2, float>), dim3(blocks(c->N)), dim3(kBlock), 0, c->stream,
+                                   c->geom, (const float *)c->FA, out);
+            }
+        } else {
+            if (c->s.precision == 64) {
+                run_flux<3, double>(c);
+                hipLaunchKernelGGL((k_gather_residual<3, double>), dim3(blocks(c->N)), dim3(kBlock), 0, c->stream,
+                                   c->geom, (const double *)c->FA, out);
+            } else {
+                run_flux<3, float>(c);
+                hipLaunchKernelGGL((k_gather_residual<3, float>), dim3(blocks(c->N)), dim3(kBlock), 0, c->stream,
+                                   c->geom, (const float *)c->FA, out);
+            }
+        }
+        hip_check(hipGetLastError(), "gather");
+        hip_check(hipMemcpyAsync(residual, out, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "D2H");
+        hip_check(hipFree(out), "hipFree");
+        if (c->timing) flush_timing(c);
+        return PSST_OK;
+    } catch (const std::exception &e) {
+        return set_error(c, e, PSST_HIP_ERROR);
+    }
+}
+
+psst_status psst_kernel_timing(psst_ctx *c, int32_t enable) {
+    if (!c) return PSST_BAD_ARG;
+    try {
+        if (!enable) flush_timing(c);
+        c->timing = enable != 0;
+        if (enable) c->stats.clear();
+        return PSST_OK;
+    } catch (const std::exception &e) {
+        return set_error(c, e, PSST_HIP_ERROR);
+    }
+}
+
+psst_status psst_kernel_stats(psst_ctx *c, const char *name, int64_t *count, double *mean_ms) {
+    if (!c || !name) return PSST_BAD_ARG;
+    try {
+        flush_timing(c);
+    } catch (const std::exception &e) {
+        return set_error(c, e, PSST_HIP_ERROR);
+    }
+    auto it = c->stats.find(name);
+    if (count) *count = it == c->stats.end() ? 0 : it->second.count;
+    if (mean_ms) *mean_ms = (it == c->stats.end() || it->second.count == 0) ? 0.0 : it->second.total_ms / it->second.count;
+    return PSST_OK;
+}
+
+psst_status psst_kernel_names(psst_ctx *c, char *buf, int32_t len) {
+    if (!c || !buf || len <= 0) return PSST_BAD_ARG;
+    std::string s;
+    for (auto &n : c->kernel_names) {
+        if (!s.empty()) s += ';';
+        s += n;
+    }
+    if ((int32_t)s.size() + 1 > len) return PSST_BAD_ARG;
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+    return PSST_OK;
+}
+
+void psst_ctx_destroy(psst_ctx *c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &p : c->pending) {
+        (void)hipEventDestroy(p.second.first);
+        (void)hipEventDestroy(p.second.second);
+    }
+    for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+}  // extern "C"

@@ -1,0 +1,161 @@
+"""Writes the golden fixtures under tests/golden/ (run in the container that
+holds the reference tree; the GPU box only reads the committed results).
+
+Fixture kinds:
+  ref_wedge80x240/   the reference's OWN committed run (wedge80_240.msh with
+                     wedge.inp/.bvp/.btq/.par/.prop -> wedge.sav,
+                     wedge_residual.dat), copied as data with LF line ends,
+                     plus a full-precision state after 100 iterations from the
+                     g++ build of the reference (oracle/_ref/psst_ref_dump).
+  c1_lc1, c1_lc2     the 40x120 15-degree wedge (generated by this package,
+                     mesh stored here), HLLC + DM/MinMod + Green-Gauss + RK4,
+                     CFL 0.8, 2000 iterations, time step modes 1 and 2: the
+                     reference build's final state (%a, full precision) and
+                     its residual file.
+  m_*                a small matrix of schemes on a jittered 10x30 wedge and on
+                     the reference's quad2D.msh, 12 iterations each.
+
+Every expected output comes from running the reference's own sources
+(built by `make -C oracle ref`), never from this package's code.
+"""
+import json
+import lzma
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+REF = "/root/reference"
+REF_DUMP = os.path.join(ROOT, "oracle", "_ref", "psst_ref_dump")
+
+
+def lf(path):
+    with open(path, "rb") as f:
+        return f.read().replace(b"\r", b"")
+
+
+def write_xz(path, data):
+    with lzma.open(path, "wb", preset=9 | lzma.PRESET_EXTREME) as f:
+        f.write(data)
+
+
+def run_reference(workdir, mesh_name, prefix):
+    with open(os.path.join(workdir, "problem_manager"), "w") as f:
+        f.write("%s\n%s\n" % (mesh_name, prefix))
+    subprocess.run([REF_DUMP, "--dump", "state.hex"], cwd=workdir, check=True, stdout=subprocess.DEVNULL,
+                   stderr=subprocess.DEVNULL)
+    with open(os.path.join(workdir, "state.hex")) as f:
+        vals = np.array([float.fromhex(l) for l in f if l.strip()])
+    with open(os.path.join(workdir, prefix + "_residual.dat")) as f:
+        res_lines = [l.rstrip("\n") for l in f]
+    return vals, res_lines
+
+
+def strip_time(lines):
+    out = []
+    for l in lines:
+        t = l.split(" ")
+        out.append(" ".join([t[0]] + t[2:]))
+    return out
+
+
+def save_expected(case_dir, vals, dim, res_lines, meta):
+    prim = vals.reshape(-1, dim + 3).T.copy()
+    np.savez_compressed(os.path.join(case_dir, "expected.npz"), prim=prim,
+                        residual=np.array(strip_time(res_lines)))
+    with open(os.path.join(case_dir, "meta.json"), "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+
+
+def ref_wedge80x240():
+    d = os.path.join(HERE, "ref_wedge80x240")
+    os.makedirs(d, exist_ok=True)
+    write_xz(os.path.join(d, "wedge80_240.msh.xz"), lf(os.path.join(REF, "wedge80_240.msh")))
+    for ext in ("inp", "bvp", "btq", "par", "prop"):
+        with open(os.path.join(d, "wedge." + ext), "wb") as f:
+            f.write(lf(os.path.join(REF, "wedge." + ext)))
+    write_xz(os.path.join(d, "wedge.sav.xz"), lf(os.path.join(REF, "wedge.sav")))
+    write_xz(os.path.join(d, "wedge_residual.dat.xz"), lf(os.path.join(REF, "wedge_residual.dat")))
+    # 100-iteration full-precision state from the reference build
+    with tempfile.TemporaryDirectory() as w:
+        with open(os.path.join(w, "wedge80_240.msh"), "wb") as f:
+            f.write(lf(os.path.join(REF, "wedge80_240.msh")))
+        for ext in ("bvp", "btq", "par", "prop"):
+            shutil.copy(os.path.join(d, "wedge." + ext), w)
+        inp = lf(os.path.join(REF, "wedge.inp")).decode().split("\n")
+        inp[3] = "100\t\t//maxIter"
+        with open(os.path.join(w, "wedge.inp"), "w") as f:
+            f.write("\n".join(inp))
+        vals, res = run_reference(w, "wedge80_240.msh", "wedge")
+    save_expected(d, vals, 2, res, {"iterations": 100, "source": "reference build, wedge.inp with maxIter=100"})
+
+
+def generated_case(name, params, case_kw, iterations, mesh_src=None):
+    import supersonicflowsolver_amd as ss
+    d = os.path.join(HERE, name)
+    os.makedirs(d, exist_ok=True)
+    with tempfile.TemporaryDirectory() as w:
+        if mesh_src is None:
+            mesh_path = os.path.join(w, "mesh.msh")
+            ss.write_wedge_msh(ss.wedge_params(**params), mesh_path)
+        else:
+            mesh_path = os.path.join(w, "mesh.msh")
+            with open(mesh_path, "wb") as f:
+                f.write(lf(mesh_src))
+        with open(mesh_path, "rb") as f:
+            write_xz(os.path.join(d, "mesh.msh.xz"), f.read())
+        kw = dict(case_kw)
+        kw["iterations"] = iterations
+        ss.write_case_files(os.path.join(w, "case"), **kw)
+        for ext in ("inp", "bvp", "btq", "par", "prop"):
+            shutil.copy(os.path.join(w, "case." + ext), d)
+        vals, res = run_reference(w, "mesh.msh", "case")
+    save_expected(d, vals, 2, res, {"iterations": iterations, "wedge_params": params, "case": kw,
+                                    "mesh_source": mesh_src or "supersonicflowsolver_amd wedge generator"})
+
+
+C1_MESH = dict(nx=120, ny=40, ramp_deg=15.0)
+C1_CASE = dict(cfl=0.8, rk_levels=4, flux=3, order=2, grad=1, limiter=2, mach=2.5, p=101325.0, T=300.0,
+               save_step=100000)
+
+SMALL = dict(nx=30, ny=10, ramp_deg=15.0, jitter=0.1, jitter_seed=7)
+MATRIX = [
+    # name, flux, order, grad, limiter, time_mode, rk
+    ("m_hll_o1_lc0_rk1", 2, 1, 1, 3, 0, 1),
+    ("m_hll_o2_gg_va_lc1_rk2", 2, 2, 1, 1, 1, 2),
+    ("m_hll_o2_lsq_mm_lc2_rk3", 2, 2, 2, 2, 2, 3),
+    ("m_hllc_o1_lc1_rk4", 3, 1, 1, 3, 1, 4),
+    ("m_hllc_o2_gg_vl_lc0_rk5", 3, 2, 1, 3, 0, 5),
+    ("m_hllc_o2_lsq_va_lc1_rk4", 3, 2, 2, 1, 1, 4),
+    ("m_hllc_o2_gg_mm_lc2_rk4", 3, 2, 1, 2, 2, 4),
+    ("m_ausm_o1_lc2_rk1", 1, 1, 1, 3, 2, 1),
+    ("m_ausm_o2_gg_mm_lc1_rk3", 1, 2, 1, 2, 1, 3),
+    ("m_ausm_o2_lsq_vl_lc0_rk2", 1, 2, 2, 3, 0, 2),
+    ("m_ausm_o2_gg_va_lc2_rk5", 1, 2, 1, 1, 2, 5),
+    ("m_hll_o2_lsq_vl_lc1_rk4", 2, 2, 2, 3, 1, 4),
+]
+
+
+def main():
+    if not os.path.exists(REF_DUMP):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    ref_wedge80x240()
+    for lc in (1, 2):
+        generated_case("c1_lc%d" % lc, C1_MESH, dict(C1_CASE, time_mode=lc), 2000)
+    for name, flux, order, grad, lim, lc, rk in MATRIX:
+        generated_case(name, SMALL, dict(cfl=0.6, rk_levels=rk, flux=flux, order=order, grad=grad, limiter=lim,
+                                         time_mode=lc, mach=2.5, save_step=100000), 12)
+    # the reference's own 40x120 mesh (18.4 degree ramp) with the C1 scheme
+    generated_case("refmesh40x120_lc2", None, dict(C1_CASE, time_mode=2), 200,
+                   mesh_src=os.path.join(REF, "wedge40_120.msh"))
+
+
+if __name__ == "__main__":
+    main()

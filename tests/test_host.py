@@ -1,0 +1,117 @@
+"""Host side of the library (CPU only): the C-ABI library loads and exports
+every entry point of include/psst_gpu.h; the host reader, grid flattening
+and settings files reproduce the oracle's setup; the generator's meshes
+round-trip through the Fluent writer and reader."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import supersonicflowsolver_amd as ss
+from _fixtures import GOLDEN, ROOT, unpack
+from oracle.oracle import Oracle
+
+
+def test_library_exports_every_declared_symbol():
+    with open(os.path.join(ROOT, "include", "psst_gpu.h")) as f:
+        text = f.read()
+    declared = set(re.findall(r"\b(psst_[a-z_0-9]+)\s*\(", text))
+    assert len(declared) >= 20
+    L = ctypes.CDLL(ss._lib.LIB_PATH)
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+    assert declared == set(ss._lib.exported_symbols())
+
+
+@pytest.mark.parametrize("name", ["ref_wedge80x240", "c1_lc1", "m_hll_o2_lsq_mm_lc2_rk3", "m_ausm_o2_lsq_vl_lc0_rk2"])
+def test_host_setup_matches_oracle(tmp_path, name):
+    mesh, prefix, _, _ = unpack(name, str(tmp_path))
+    case = ss.Case.open(mesh, prefix)
+    o = Oracle(mesh, prefix)
+    assert (case.dim, case.n_cells, case.n_ghosts, case.n_faces) == (o.dim, o.n_cells, o.n_ghosts, o.n_faces)
+    assert np.array_equal(case.initial_state(), o.state())
+    s = case.settings()
+    assert (s.gamma, s.rm, s.cp, s.cv, s.cfl) == (o.gamma, o.rm, o.cp, o.cv, o.cfl)
+    assert s.rk_levels == o.rk_levels
+    if s.time_mode == 0:
+        assert s.uniform_dt == o.uniform_dt
+    assert case.mesh().min_volume == o.min_vol
+
+
+def test_generator_roundtrip(tmp_path):
+    p = ss.wedge_params(nx=40, ny=12, jitter=0.1, numbering=2)
+    path = str(tmp_path / "w.msh")
+    ss.write_wedge_msh(p, path)
+    ss.write_case_files(str(tmp_path / "c"))
+    a = ss.Case.open(path, str(tmp_path / "c")).raw()
+    b = ss.Case.wedge(p, str(tmp_path / "c")).raw()
+    for k in a:
+        if k == "zone_names":
+            assert a[k] == b[k]
+        else:
+            assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("numbering", [0, 1, 2])
+def test_generator_mesh_is_closed_and_oriented(tmp_path, numbering):
+    """Every interior cell's faces close (sum of n A = 0), volumes are
+    positive and sum to the channel area, and each normal points toward
+    neighbour 0 (from centroid 1 to centroid 0)."""
+    p = ss.wedge_params(nx=30, ny=10, jitter=0.1, numbering=numbering)
+    ss.write_case_files(str(tmp_path / "c"))
+    case = ss.Case.wedge(p, str(tmp_path / "c"))
+    m = case.mesh()
+    N, G, F = case.n_cells, case.n_ghosts, case.n_faces
+    NT = N + G
+    vol = np.ctypeslib.as_array(m.cell_volume, shape=(NT,))
+    cx = np.ctypeslib.as_array(m.cell_centre, shape=(3 * NT,)).reshape(3, NT)
+    fn = np.ctypeslib.as_array(m.face_normal, shape=(3 * F,)).reshape(3, F)
+    fa = np.ctypeslib.as_array(m.face_area, shape=(F,))
+    fc = np.ctypeslib.as_array(m.face_cells, shape=(2 * F,)).reshape(2, F)
+    assert (vol[:N] > 0).all() and (vol[N:] == 0).all()
+    area = 3.0 * 1.0 - 0.5 * 0.6 * 0.6 * np.tan(np.radians(15)) - 1.9 * 0.6 * np.tan(np.radians(15))
+    assert vol[:N].sum() == pytest.approx(area, rel=1e-12)
+    acc = np.zeros((2, NT))
+    for k in range(2):
+        np.add.at(acc[k], fc[0], -fn[k] * fa)
+        np.add.at(acc[k], fc[1], fn[k] * fa)
+    assert np.abs(acc[:, :N]).max() < 1e-14
+    d = cx[:2, fc[0]] - cx[:2, fc[1]]
+    assert ((d * fn[:2]).sum(axis=0) > 0).all()
+
+
+def test_bad_inputs_raise(tmp_path):
+    with pytest.raises(ss.PsstError):
+        ss.Case.open(str(tmp_path / "missing.msh"), str(tmp_path / "c"))
+    mesh, prefix, _, _ = unpack("m_hll_o1_lc0_rk1", str(tmp_path))
+    with open(prefix + ".bvp", "a") as f:
+        f.write("\nnosuchzone\t15")
+    with pytest.raises(ss.PsstError, match="zone"):
+        ss.Case.open(mesh, prefix)
+
+
+def test_trailing_newline_in_boundary_files_is_accepted(tmp_path):
+    """The reference crashes on a trailing newline in .bvp/.btq (quirk Q11);
+    the host reader accepts it and reads the same boundary set."""
+    mesh, prefix, _, _ = unpack("m_hll_o1_lc0_rk1", str(tmp_path))
+    a = ss.Case.open(mesh, prefix).initial_state()
+    for ext in ("bvp", "btq"):
+        with open(prefix + "." + ext, "a") as f:
+            f.write("\n")
+    b = ss.Case.open(mesh, prefix).initial_state()
+    assert np.array_equal(a, b)
+
+
+def test_crlf_mesh_is_accepted(tmp_path):
+    """CRLF line ends hang the reference reader (quirk Q12); the host reader
+    parses by structure."""
+    mesh, prefix, _, _ = unpack("m_hll_o1_lc0_rk1", str(tmp_path))
+    with open(mesh, "rb") as f:
+        data = f.read()
+    with open(mesh + ".crlf", "wb") as f:
+        f.write(data.replace(b"\n", b"\r\n"))
+    a = ss.Case.open(mesh, prefix).raw()
+    b = ss.Case.open(mesh + ".crlf", prefix).raw()
+    assert np.array_equal(a["xyz"], b["xyz"]) and np.array_equal(a["face_v"], b["face_v"])
