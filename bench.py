@@ -1,0 +1,205 @@
+"""Benchmark of the explicit finite-volume Euler step on one MI355X.
+
+Workload (the reference path at scale): 2D supersonic compression-ramp
+channel, domain 3 x 1 with a 15 degree ramp from x = 0.5 to 1.1, quad mesh
+3840 x 1280 = 4,915,200 cells, interior nodes jittered +-10% of the local
+spacing (seed 1), cells numbered in reverse Cuthill-McKee order; inflow
+M 2.5, p 101325 Pa, T 300 K, gamma 1.4; HLLC flux, DM/MUSCL reconstruction
+with the MinMod limiter, Green-Gauss gradients, 4-stage RK, local time step
+(mode 2) at CFL 0.8.  A step is one solver iteration (one gradient pass and
+four RK stages over every cell).  Inputs are resident in HBM before timing.
+
+Prints one JSON line: cell-iterations/s (unit "cell-updates/s", one update
+= one full iteration of a cell), the dominant kernel's achieved HBM
+bandwidth against the 8 TB/s peak (HIP events on the solver's stream), and
+the CPU oracle's serial throughput on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s (% HBM roofline); time-to-1e-8 residual: NN warm vs plain"
+HBM_PEAK_GBS = 8000.0
+FLUX_IDS = {"ausm": 1, "hll": 2, "hllc": 3}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=3840)
+    ap.add_argument("--ny", type=int, default=1280)
+    ap.add_argument("--flux", default="hllc", choices=sorted(FLUX_IDS))
+    ap.add_argument("--precision", type=int, default=64, choices=(32, 64))
+    ap.add_argument("--numbering", default="rcm", choices=("rcm", "random", "column"))
+    ap.add_argument("--time-mode", type=int, default=2)
+    ap.add_argument("--fused", type=int, default=-1, help="-1: library default")
+    ap.add_argument("--cpu-iters", type=int, default=2, help="iterations of the CPU oracle sample (0: skip)")
+    ap.add_argument("--also-fp32", type=int, default=1, help="also time the FP32 mode")
+    ap.add_argument("--timing-iters", type=int, default=10)
+    ap.add_argument("--profile", action="store_true", help="short run for rocprof (no CPU baseline, no JSON extras)")
+    return ap.parse_args()
+
+
+def alg_bytes(kind, N, G, F, nnz, dim, b, last_stage=False):
+    """Algorithmic (unique) HBM bytes of one launch: every array the kernel
+    must read or write, touched once.  b = state bytes, geometry 8, index 4."""
+    nv = dim + 2  # u..., p, T
+    if kind == "flux":  # face-based flux, second order
+        rd = N * nv * b + G * (nv + 1) * b + N * nv * dim * b + N * dim * 8 + F * ((dim + 1) * 8 + 2 * 4)
+        return rd + F * nv * b
+    if kind == "update":
+        rd = F * nv * b + N * 4 + nnz * 4 + F * 4 + N * (dim + 1) * b + N * dim * 8 + N * 8 + N * nv * b
+        wr = N * (nv + 1) * b + (N * nv * b if last_stage else 0)
+        return rd + wr
+    if kind == "stage":  # fused stage: flux + gather + update + ghosts
+        rd = (N * (nv + 1) * b + G * (nv + 1) * b + N * nv * dim * b + N * dim * 8 + F * ((dim + 1) * 8 + 2 * 4)
+              + N * 4 + nnz * 4 + N * dim * 8 + N * 8 + N * nv * b)
+        wr = (N + G) * (nv + 1) * b + (N * nv * b if last_stage else 0)
+        return rd + wr
+    if kind == "grad":
+        rd = (N + G) * nv * b + F * 8 + F * (dim + 2) * 8 + N * 8 + N * 4 + nnz * 4
+        return rd + N * nv * dim * b
+    if kind == "bc":
+        return G * (nv + 1) * b * 2 + G * 12
+    raise ValueError(kind)
+
+
+def survey_bytes_per_cell_iteration(dim, b, N, F, stages=4):
+    """Unique-touch model per cell-iteration (second order, RK): per stage
+    q read + grad read + w0 read + q write + V, x; faces n, A, c0, c1; plus
+    the gradient pass and the w0 snapshot."""
+    nv = dim + 2
+    stage = N * (nv * b + nv * dim * b + nv * b + nv * b + (1 + dim) * 8) + F * ((dim + 1) * 8 + 2 * 4)
+    grad = N * (nv * b + nv * dim * b + 8) + F * ((dim + 1) * 8 + 8 + 8)
+    snap = N * nv * b
+    return (stages * stage + grad + snap) / N
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the path's multi-GPU sharding is a design note); "
+              "--gpus %d is not supported" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import supersonicflowsolver_amd as ss
+
+    numbering = {"column": 0, "rcm": 1, "random": 2}[a.numbering]
+    tmp = tempfile.mkdtemp(prefix="psst_bench_")
+    prefix = os.path.join(tmp, "case")
+    ss.write_case_files(prefix, cfl=0.8, time_mode=a.time_mode, iterations=a.steps, rk_levels=4,
+                        flux=FLUX_IDS[a.flux], order=2, grad=1, limiter=2, mach=2.5, p=101325.0, T=300.0)
+    params = ss.wedge_params(nx=a.nx, ny=a.ny, ramp_deg=15.0, jitter=0.1, jitter_seed=1, numbering=numbering,
+                             perm_seed=2)
+    t0 = time.perf_counter()
+    case = ss.Case.wedge(params, prefix)
+    t_setup = time.perf_counter() - t0
+    N, G, F = case.n_cells, case.n_ghosts, case.n_faces
+    nnz = int(case.mesh().visit_offsets[N])
+    print("bench: mesh %dx%d, %d cells, %d faces, host setup %.1f s" % (a.nx, a.ny, N, F, t_setup),
+          file=sys.stderr, flush=True)
+
+    def timed(precision):
+        kw = dict(precision=precision)
+        if a.fused >= 0:
+            kw["fused"] = a.fused
+        s = ss.Solver(case, **kw)
+        s.run(a.warmup)
+        t = time.perf_counter()
+        s.run(a.steps)
+        el = time.perf_counter() - t
+        # per-kernel mean durations (HIP events on the solver's stream)
+        s.kernel_timing(True)
+        s.run(a.timing_iters)
+        stats = {}
+        for name in s.kernel_names():
+            cnt, ms = s.kernel_stats(name)
+            if cnt:
+                stats[name] = (cnt, ms)
+        s.kernel_timing(False)
+        state = s.state()
+        s.close()
+        return el, stats, state
+
+    print("bench: FP%d, %d warmup + %d timed iterations" % (a.precision, a.warmup, a.steps), file=sys.stderr,
+          flush=True)
+    el, stats, state = timed(a.precision)
+    if a.profile:
+        print("bench: done (%.3f s)" % el, file=sys.stderr, flush=True)
+        return
+    if not np.isfinite(state).all():
+        raise SystemExit("bench: non-finite state")
+    b = 8 if a.precision == 64 else 4
+    cell_its = N * a.steps / el
+    # dominant kernel by total time per iteration
+    per_iter = {k: cnt * ms / a.timing_iters for k, (cnt, ms) in stats.items()}
+    dom = max(per_iter, key=per_iter.get)
+    kind = {"flux": "flux", "update": "update", "grad_gg": "grad", "grad_lsq": "grad", "bc": "bc",
+            "stage": "stage", "stage_last": "stage"}.get(dom, dom)
+    nbytes = alg_bytes(kind, N, G, F, nnz, 2, b, last_stage=(dom == "stage_last"))
+    mean_ms = stats[dom][1]
+    achieved = nbytes / (mean_ms * 1e-3) / 1e9
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "alg_bytes_per_launch": int(nbytes), "mean_ms": round(mean_ms, 4)}
+    whole = survey_bytes_per_cell_iteration(2, b, N, F)
+    out = {
+        "metric": METRIC, "value": round(cell_its, 1), "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(el / a.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if a.precision == 64 else "f32",
+        "data": "synthetic (generated wedge mesh, freestream initial state)",
+        "config": {"workload": "2D wedge %dx%d quads (%d cells), jitter 10%% seed 1, %s numbering; %s + DM/MinMod + "
+                               "Green-Gauss, RK4, time step mode %d, CFL 0.8" % (a.nx, a.ny, N, a.numbering,
+                                                                              a.flux.upper(), a.time_mode),
+                   "cells": N, "faces": F, "rk_stages": 4, "precision": a.precision},
+        "roofline": roofline,
+        "cell_stage_updates_per_s": round(cell_its * 4, 1),
+        "whole_step_model": {"bytes_per_cell_iteration": round(whole, 1),
+                             "achieved_GBs": round(whole * cell_its / 1e9, 1),
+                             "frac": round(whole * cell_its / 1e9 / HBM_PEAK_GBS, 4)},
+        "kernels_ms": {k: round(v[1], 4) for k, v in stats.items()},
+    }
+    if a.also_fp32 and a.precision == 64:
+        el32, stats32, state32 = timed(32)
+        ci32 = N * a.steps / el32
+        whole32 = survey_bytes_per_cell_iteration(2, 4, N, F)
+        err = {}
+        for k, nm in enumerate(["u", "v", "p", "T", "rho"]):
+            err[nm] = float(np.linalg.norm(state32[k, :N] - state[k, :N]) / np.linalg.norm(state[k, :N]))
+        out["fp32"] = {"value": round(ci32, 1), "ms_per_step": round(el32 / a.steps * 1e3, 4),
+                       "whole_step_frac": round(whole32 * ci32 / 1e9 / HBM_PEAK_GBS, 4),
+                       "kernels_ms": {k: round(v[1], 4) for k, v in stats32.items()},
+                       "rel_l2_vs_fp64_after_steps": err}
+    if a.cpu_iters > 0:
+        from oracle.oracle import Oracle
+        raw = case.raw()
+        raw["dim"] = 2
+        t = time.perf_counter()
+        o = Oracle(None, prefix, raw=raw)
+        t_open = time.perf_counter() - t
+        t = time.perf_counter()
+        o.run(a.cpu_iters)
+        el_cpu = time.perf_counter() - t
+        try:
+            cpu_model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+        except Exception:
+            cpu_model = "unknown"
+        out["cpu_baseline"] = {"value": round(N * a.cpu_iters / el_cpu, 1), "unit": "cell-updates/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "%d iterations of the same %d-cell FP64 workload, serial oracle on one "
+                                         "host core (%s); setup %.1f s not timed" % (a.cpu_iters, N, cpu_model,
+                                                                                      t_open)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -31,6 +31,11 @@ def lib():
         L.oracle_open.argtypes = [C.c_char_p, C.c_char_p]
         L.oracle_last_error.restype = C.c_char_p
         L.oracle_close.argtypes = [C.c_void_p]
+        L.oracle_open_raw.restype = C.c_void_p
+        P = C.POINTER
+        L.oracle_open_raw.argtypes = [C.c_int, C.c_long, P(C.c_double), C.c_long, P(C.c_int8), P(C.c_long),
+                                      P(C.c_long), P(C.c_long), C.c_long, P(C.c_long), C.c_long, P(C.c_long),
+                                      P(C.c_long), P(C.c_long), C.c_char_p, C.c_char_p]
         for n in ("oracle_info",):
             getattr(L, n).argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         for n in ("oracle_constants", "oracle_get_state", "oracle_set_state", "oracle_get_w", "oracle_gradients",
@@ -54,9 +59,27 @@ def _dp(a):
 class Oracle:
     """The reference's explicit_solve restated on the CPU (FP64, serial)."""
 
-    def __init__(self, mesh_path, case_prefix):
+    def __init__(self, mesh_path, case_prefix, raw=None):
+        """Opens a Fluent mesh file, or (mesh_path=None) a raw mesh dict as
+        returned by supersonicflowsolver_amd.Case.raw()."""
         L = lib()
-        h = L.oracle_open(str(mesh_path).encode(), str(case_prefix).encode())
+        if raw is None:
+            h = L.oracle_open(str(mesh_path).encode(), str(case_prefix).encode())
+        else:
+            P = C.POINTER
+            r = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in raw.items()}
+            pl = lambda a: np.ascontiguousarray(a, dtype=np.int64).ctypes.data_as(P(C.c_long))
+            keep = [np.ascontiguousarray(r[k], dtype=np.int64) for k in
+                    ("face_v", "face_c", "face_zone", "cell_zone", "zone_id", "zone_range", "zone_bc")]
+            ptr = [a.ctypes.data_as(P(C.c_long)) for a in keep]
+            xyz = np.ascontiguousarray(r["xyz"], dtype=np.float64)
+            fnv = np.ascontiguousarray(r["face_nv"], dtype=np.int8)
+            dim = 3 if np.any(xyz[:, 2] != 0) else 2
+            if "dim" in raw:
+                dim = raw["dim"]
+            h = L.oracle_open_raw(dim, xyz.shape[0], _dp(xyz), fnv.size, fnv.ctypes.data_as(P(C.c_int8)),
+                                  ptr[0], ptr[1], ptr[2], keep[3].size, ptr[3], keep[4].size, ptr[4], ptr[5],
+                                  ptr[6], "\n".join(r["zone_names"]).encode(), str(case_prefix).encode())
         if not h:
             raise RuntimeError("oracle: " + L.oracle_last_error().decode())
         self._h = C.c_void_p(h)

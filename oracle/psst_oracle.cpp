@@ -1393,6 +1393,53 @@ void *oracle_open(const char *mesh_path, const char *case_prefix) {
     }
 }
 
+/* Same as oracle_open for a mesh given as arrays (the content of a Fluent
+ * file): xyz [nv][3], face_nv [nf], face_v [nf][4], face_c [nf][2],
+ * face_zone [nf], cell_zone [nc], zones (file order) with id, first/last,
+ * bc and '\n'-separated names. */
+void *oracle_open_raw(int dim, long nv, const double *xyz, long nf, const signed char *face_nv, const long *face_v,
+                      const long *face_c, const long *face_zone, long nc, const long *cell_zone, long nz,
+                      const long *zone_id, const long *zone_range, const long *zone_bc, const char *zone_names,
+                      const char *case_prefix) {
+    try {
+        RawMesh m;
+        m.dim = dim;
+        m.xyz.assign(xyz, xyz + 3 * nv);
+        m.faces.resize((size_t)nf);
+        for (long i = 0; i < nf; ++i) {
+            RawFace &f = m.faces[(size_t)i];
+            f.nv = face_nv[i];
+            for (int k = 0; k < 4; ++k) f.v[k] = face_v[4 * i + k];
+            f.c[0] = face_c[2 * i];
+            f.c[1] = face_c[2 * i + 1];
+            f.zone = face_zone[i];
+        }
+        m.cell_zone.assign(cell_zone, cell_zone + nc);
+        std::string names(zone_names);
+        size_t pos = 0;
+        for (long z = 0; z < nz; ++z) {
+            size_t e = names.find('\n', pos);
+            if (e == std::string::npos) e = names.size();
+            RawZone rz;
+            rz.id = zone_id[z];
+            rz.first = zone_range[2 * z];
+            rz.last = zone_range[2 * z + 1];
+            rz.bc = zone_bc[z];
+            rz.name = names.substr(pos, e - pos);
+            pos = e + 1;
+            m.zone_pos[rz.id] = m.zones.size();
+            m.zones.push_back(rz);
+        }
+        auto *o = new Oracle();
+        build_grid(*o, m);
+        solver_parameters(*o, case_prefix);
+        return o;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return nullptr;
+    }
+}
+
 void oracle_close(void *h) { delete static_cast<Oracle *>(h); }
 
 int oracle_info(void *h, int *info /* [8] */) {

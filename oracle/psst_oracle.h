@@ -8,6 +8,10 @@ extern "C" {
 #endif
 const char *oracle_last_error(void);
 void *oracle_open(const char *mesh_path, const char *case_prefix);
+void *oracle_open_raw(int dim, long nv, const double *xyz, long nf, const signed char *face_nv, const long *face_v,
+                      const long *face_c, const long *face_zone, long nc, const long *cell_zone, long nz,
+                      const long *zone_id, const long *zone_range, const long *zone_bc, const char *zone_names,
+                      const char *case_prefix);
 void oracle_close(void *h);
 int oracle_info(void *h, int *info);          /* dim, N, G, F, rk, iter, max_iter, iter_step */
 int oracle_constants(void *h, double *c);     /* gamma, rm, cp, cv, ref_vel, cfl, min_vol, uniform_dt */
