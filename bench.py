@@ -45,6 +45,11 @@ def parse():
     ap.add_argument("--also-fp32", type=int, default=1, help="also time the FP32 mode")
     ap.add_argument("--timing-iters", type=int, default=10)
     ap.add_argument("--profile", action="store_true", help="short run for rocprof (no CPU baseline, no JSON extras)")
+    ap.add_argument("--tile-cells", type=int, default=0)
+    ap.add_argument("--tile-threads", type=int, default=0)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="profile mode: first copy a buffer of this many MiB (a kernel of known traffic that "
+                         "calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -108,10 +113,23 @@ def main():
     print("bench: mesh %dx%d, %d cells, %d faces, host setup %.1f s" % (a.nx, a.ny, N, F, t_setup),
           file=sys.stderr, flush=True)
 
+    if a.profile and a.calib_copy_mb > 0:
+        import ctypes
+        ms = ctypes.c_double(0)
+        rc = ss._lib.lib().psst_copy_calibration(a.calib_copy_mb << 20, 3, ctypes.byref(ms))
+        if rc != 0:
+            raise SystemExit("bench: calibration copy failed")
+        print("bench: calibration copy (k_copy) of %d MiB: %.3f ms, %.0f GB/s" %
+              (a.calib_copy_mb, ms.value, 2 * (a.calib_copy_mb << 20) / ms.value / 1e6), file=sys.stderr, flush=True)
+
     def timed(precision):
         kw = dict(precision=precision)
         if a.fused >= 0:
             kw["fused"] = a.fused
+        if a.tile_cells:
+            kw["tile_cells"] = a.tile_cells
+        if a.tile_threads:
+            kw["tile_threads"] = a.tile_threads
         s = ss.Solver(case, **kw)
         s.run(a.warmup)
         t = time.perf_counter()

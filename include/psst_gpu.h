@@ -112,10 +112,12 @@ typedef struct psst_settings {
     int32_t precision;            /* 64: FP64 state (parity); 32: FP32 state/flux, FP64 geometry and norms */
     int32_t dt_ordering;          /* 0: reference order (PSST_DT_LOCAL1 sees neighbours already updated
                                      in the same stage, psstFiniteVolumeGrid.cpp:317-347); 1: Jacobi */
-    int32_t fused;                /* 0: face-flux + cell-gather kernels; 1: fused cell-centric stage kernel */
+    int32_t fused;                /* 0: face-flux + cell-gather kernels; 1: fused tiled stage kernel */
     int32_t n_bc;                 /* entries of bc */
     const psst_bc *bc;
     double venkat_k;              /* Venkatakrishnan constant K (limiter 4 only) */
+    int32_t tile_cells;           /* fused kernels: cells per tile (0 = default 256) */
+    int32_t tile_threads;         /* fused kernels: threads per tile workgroup, 256 or 512 (0 = default) */
 } psst_settings;
 
 typedef struct psst_ctx psst_ctx;
@@ -166,6 +168,12 @@ psst_status psst_kernel_timing(psst_ctx *ctx, int32_t enable);
 psst_status psst_kernel_stats(psst_ctx *ctx, const char *name, int64_t *count, double *mean_ms);
 /* Names the kernels a psst_run issues (";"-separated) into buf. */
 psst_status psst_kernel_names(psst_ctx *ctx, char *buf, int32_t buf_len);
+
+/* Diagnostic: times a plain device-to-device copy kernel ("k_copy") moving
+ * `bytes` (read) + `bytes` (write), reps times; used to calibrate hardware
+ * traffic counters against a kernel of known traffic.  No reference
+ * counterpart. */
+psst_status psst_copy_calibration(int64_t bytes, int32_t reps, double *mean_ms);
 
 void psst_ctx_destroy(psst_ctx *ctx);
 

@@ -41,6 +41,7 @@ class Settings(C.Structure):
         ("cfl", C.c_double), ("gamma", C.c_double), ("rm", C.c_double), ("cp", C.c_double), ("cv", C.c_double),
         ("uniform_dt", C.c_double), ("precision", C.c_int32), ("dt_ordering", C.c_int32), ("fused", C.c_int32),
         ("n_bc", C.c_int32), ("bc", C.POINTER(BC)), ("venkat_k", C.c_double),
+        ("tile_cells", C.c_int32), ("tile_threads", C.c_int32),
     ]
 
 
@@ -69,6 +70,7 @@ _SIGS = {
     "psst_kernel_stats": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "psst_kernel_names": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
     "psst_ctx_destroy": (None, [C.c_void_p]),
+    "psst_copy_calibration": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "psst_last_error": (C.c_char_p, [C.c_void_p]),
     "psst_case_open": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     "psst_wedge_write_msh": (C.c_int, [C.POINTER(WedgeParams), C.c_char_p]),

@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "psst_gpu.h"
+#include "tiles.hpp"
 
 namespace {
 
@@ -63,6 +64,7 @@ struct Geom {
     const int *__restrict__ gface;    // [G]
     const int *__restrict__ gnb;      // [G]
     const int *__restrict__ gbc;      // [G]
+    const int *__restrict__ rank;     // [NT] position in the reference's update loop (ghosts: >= N)
 };
 
 struct BCDev {
@@ -74,12 +76,21 @@ struct Phys {
     double gamma, rm, cp, cv, cfl, dt_uniform, venkat_k;
 };
 
+// Gradients are stored cell-major ("AoS"): the (dim+2) x dim components of a
+// cell are contiguous, padded to an even count so that a cell's gradients
+// are 16-byte aligned and load as a few wide accesses.
+template <int DIM>
+struct GradLayout {
+    static constexpr int NC = (DIM + 2) * DIM;       // components per cell
+    static constexpr int STRIDE = (NC + 1) & ~1;     // padded
+};
+
 template <typename T>
 struct State {
     const T *__restrict__ q;  // [(dim+3)][NT]  stage-start primitives
     T *__restrict__ qn;       // [(dim+3)][NT]  new primitives
     T *__restrict__ W0;       // [(dim+2)][N]   conservative state at iteration start
-    T *__restrict__ grad;     // [(dim+2)][dim][N]
+    T *__restrict__ grad;     // [N][stride]: component (var i, axis k) at i*dim + k
     T *__restrict__ FA;       // [(dim+2)][F]
 };
 
@@ -162,7 +173,7 @@ __global__ void __launch_bounds__(kBlock) k_grad_gg(Geom g, State<T> s) {
 #pragma unroll
     for (int n = 0; n < NV; ++n)
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) s.grad[((size_t)n * DIM + k) * g.N + c] = acc[n][k];
+        for (int k = 0; k < DIM; ++k) s.grad[(size_t)c * GradLayout<DIM>::STRIDE + n * DIM + k] = acc[n][k];
 }
 
 // psstLeastSquare::calculate (psstGradCalculation.cpp:268-368), per interior
@@ -198,7 +209,7 @@ __global__ void __launch_bounds__(kBlock) k_grad_lsq(Geom g, State<T> s) {
 #pragma unroll
     for (int n = 0; n < NV; ++n)
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) s.grad[((size_t)n * DIM + k) * g.N + c] = acc[n][k];
+        for (int k = 0; k < DIM; ++k) s.grad[(size_t)c * GradLayout<DIM>::STRIDE + n * DIM + k] = acc[n][k];
 }
 
 // ---------------------------------------------------------------------------
@@ -209,14 +220,12 @@ struct Side {
     T u[DIM], p, T_, rho;
 };
 
-// psstReconstructorDM::reconstruct_vals (psstValueReconstructor.cpp:70-134)
-template <int DIM, typename T, int LIM>
-__device__ __forceinline__ void reconstruct_dm(const Geom &g, const State<T> &s, int a0, int a1, Side<DIM, T> &r0,
-                                               Side<DIM, T> &r1, double rm) {
+// psstReconstructorDM::reconstruct_vals (psstValueReconstructor.cpp:70-134).
+// G0/G1 return the gradient component (var i, axis k) of neighbour 0 / 1;
+// RLR = x_0 - x_1.
+template <int DIM, typename T, int LIM, typename G0, typename G1>
+__device__ __forceinline__ void reconstruct_dm(Side<DIM, T> &r0, Side<DIM, T> &r1, const T *RLR, T rm, G0 g0, G1 g1) {
     constexpr int NV = DIM + 2;
-    T RLR[DIM];
-#pragma unroll
-    for (int k = 0; k < DIM; ++k) RLR[k] = T(g.cx[(size_t)k * g.NT + a0] - g.cx[(size_t)k * g.NT + a1]);
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
         T *p0 = (i < DIM) ? &r0.u[i] : (i == DIM ? &r0.p : &r0.T_);
@@ -224,8 +233,8 @@ __device__ __forceinline__ void reconstruct_dm(const Geom &g, const State<T> &s,
         T dl = T(0), dr = T(0);
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
-            dl += s.grad[((size_t)i * DIM + k) * g.N + a1] * RLR[k];
-            dr += s.grad[((size_t)i * DIM + k) * g.N + a0] * RLR[k];
+            dl += g1(i, k) * RLR[k];
+            dr += g0(i, k) * RLR[k];
         }
         const T v0 = *p0, v1 = *p1;
         T vlm = v1 - (T(2) * dl - (v0 - v1));
@@ -239,8 +248,8 @@ __device__ __forceinline__ void reconstruct_dm(const Geom &g, const State<T> &s,
         *p0 = v0 + T(0.5) * limiter<T, LIM>(nr, dnr) * dnr;
         *p1 = v1 + T(0.5) * limiter<T, LIM>(nl, dnl) * dnl;
     }
-    r0.rho = r0.p / (T(rm) * r0.T_);
-    r1.rho = r1.p / (T(rm) * r1.T_);
+    r0.rho = r0.p / (rm * r0.T_);
+    r1.rho = r1.p / (rm * r1.T_);
 }
 
 template <int DIM, typename T>
@@ -250,6 +259,158 @@ __device__ __forceinline__ void load_side(const State<T> &s, int NT, int a, Side
     r.p = s.q[(size_t)DIM * NT + a];
     r.T_ = s.q[(size_t)(DIM + 1) * NT + a];
     r.rho = s.q[(size_t)(DIM + 2) * NT + a];
+}
+
+// Boundary face: physical flux of the ghost state (quirk Q5),
+// psstConvectiveHLL::calculate :139-201 (same in HLLC and AUSM).
+template <int DIM, typename T>
+__device__ __forceinline__ void ghost_flux(const Side<DIM, T> &gs, const T *nrm, T cp, T *flow) {
+    T vn = T(0), vsq = T(0);
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+        vn += gs.u[k] * nrm[k];
+        vsq += gs.u[k] * gs.u[k];
+    }
+    T H = cp * gs.T_ + T(0.5) * vsq;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) flow[k] = gs.rho * gs.u[k] * vn + gs.p * nrm[k];
+    flow[DIM] = gs.rho * vn;
+    flow[DIM + 1] = gs.rho * vn * H;
+}
+
+// Physical flux F(q).n and conservative state of one side
+// (psstConvectiveHLL::calculate :235-244).
+template <int DIM, typename T>
+__device__ __forceinline__ void side_flux(const Side<DIM, T> &r, T vn, T H, const T *nrm, T *fi) {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) fi[k] = r.rho * r.u[k] * vn + r.p * nrm[k];
+    fi[DIM] = r.rho * vn;
+    fi[DIM + 1] = r.rho * vn * H;
+}
+template <int DIM, typename T>
+__device__ __forceinline__ void side_state(const Side<DIM, T> &r, T E, T *wi) {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) wi[k] = r.rho * r.u[k];
+    wi[DIM] = r.rho;
+    wi[DIM + 1] = r.rho * E;
+}
+
+// Interior face: Riemann flux of the two (reconstructed) states, r[0] on the
+// side the normal points to.  HLL psstConvectiveFlow.cpp:215-286, HLLC
+// :499-585, AUSM :782-851.  Each branch forms only the side fluxes and
+// states it uses; every value it forms is computed exactly as the reference
+// computes it.
+template <int DIM, typename T, int FLUX>
+__device__ __forceinline__ void riemann_flux(const Side<DIM, T> (&r)[2], const T *nrm, const Phys &ph, T *flow) {
+    constexpr int WS = DIM + 2;
+    const T cp = T(ph.cp), cv = T(ph.cv), rm = T(ph.rm), gam = T(ph.gamma);
+    T vn[2], a[2], H[2], vsq2[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        vn[j] = T(0);
+        T vsq = T(0);
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            vn[j] += r[j].u[k] * nrm[k];
+            vsq += r[j].u[k] * r[j].u[k];
+        }
+        vsq2[j] = vsq;
+        a[j] = dsqrt(gam * rm * r[j].T_);
+        H[j] = cp * r[j].T_ + T(0.5) * vsq;
+    }
+    if (FLUX == PSST_FLUX_HLL || FLUX == PSST_FLUX_HLLC) {
+        T Hr = roe_avg(H[1], H[0], r[1].rho, r[0].rho);
+        T vrn = T(0), vsq = T(0);
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            T vr = roe_avg(r[1].u[k], r[0].u[k], r[1].rho, r[0].rho);
+            vrn += vr * nrm[k];
+            vsq += vr * vr;
+        }
+        T ar = dsqrt((gam - T(1)) * (Hr - T(0.5) * vsq));
+        T s1 = vn[1] - a[1];
+        T tmp = vrn - ar;
+        if (s1 > tmp) s1 = tmp;
+        T s0 = vn[0] + a[0];
+        tmp = vrn + ar;
+        if (s0 < tmp) s0 = tmp;
+        if (s1 >= T(0)) {
+            side_flux<DIM, T>(r[1], vn[1], H[1], nrm, flow);
+            return;
+        }
+        if (FLUX == PSST_FLUX_HLL) {
+            if (s1 <= T(0) && s0 >= T(0)) {
+                T fi0[WS], fi1[WS], wi0[WS], wi1[WS];
+                side_flux<DIM, T>(r[0], vn[0], H[0], nrm, fi0);
+                side_flux<DIM, T>(r[1], vn[1], H[1], nrm, fi1);
+                side_state<DIM, T>(r[0], cv * r[0].T_ + T(0.5) * vsq2[0], wi0);
+                side_state<DIM, T>(r[1], cv * r[1].T_ + T(0.5) * vsq2[1], wi1);
+#pragma unroll
+                for (int j = 0; j < WS; ++j)
+                    flow[j] = (s0 * fi1[j] - s1 * fi0[j] + s0 * s1 * (wi0[j] - wi1[j])) / (s0 - s1);
+            } else {
+                // s0 <= 0, or NaN speeds (the reference throws, quirk Q13; the
+                // NaN propagates into the state and psst_run reports it)
+                side_flux<DIM, T>(r[0], vn[0], H[0], nrm, flow);
+            }
+            return;
+        }
+        T sh = (r[0].p - r[1].p + r[1].rho * vn[1] * (s1 - vn[1]) - r[0].rho * vn[0] * (s0 - vn[0])) /
+               (r[1].rho * (s1 - vn[1]) - r[0].rho * (s0 - vn[0]));
+        if ((s1 <= T(0) && sh >= T(0)) || (sh <= T(0) && s0 >= T(0))) {
+            const int K = (s1 <= T(0) && sh >= T(0)) ? 1 : 0;
+            const Side<DIM, T> &rk = r[K];
+            const T sK = K ? s1 : s0;
+            const T vK = vn[K];
+            T fiK[WS], wiK[WS];
+            side_flux<DIM, T>(rk, vK, H[K], nrm, fiK);
+            side_state<DIM, T>(rk, cv * rk.T_ + T(0.5) * vsq2[K], wiK);
+            const T tt = sK - sh;
+            const T PK = rk.p + rk.rho * (sK - vK) * (sh - vK);
+#pragma unroll
+            for (int j = 0; j < DIM; ++j) flow[j] = (sh * (sK * wiK[j] - fiK[j]) + sK * PK * nrm[j]) / tt;
+            flow[DIM] = (sh * (sK * wiK[DIM] - fiK[DIM])) / tt;
+            flow[DIM + 1] = (sh * (sK * wiK[DIM + 1] - fiK[DIM + 1]) + sK * PK * sh) / tt;
+        } else {
+            side_flux<DIM, T>(r[0], vn[0], H[0], nrm, flow);
+        }
+    } else {
+        T M[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) M[j] = vn[j] / a[j];
+        T Ms1, ps1, Ms0, ps0;
+        if (M[1] >= T(1)) {
+            Ms1 = M[1];
+            ps1 = r[1].p;
+        } else if (M[1] <= T(-1)) {
+            Ms1 = T(0);
+            ps1 = T(0);
+        } else {
+            Ms1 = T(0.25) * (M[1] + T(1)) * (M[1] + T(1));
+            ps1 = T(0.25) * r[1].p * (M[1] + T(1)) * (M[1] + T(1)) * (T(2) - M[1]);
+        }
+        if (M[0] >= T(1)) {
+            Ms0 = T(0);
+            ps0 = T(0);
+        } else if (M[0] <= T(-1)) {
+            Ms0 = M[0];
+            ps0 = r[0].p;
+        } else {
+            Ms0 = T(-0.25) * (M[0] - T(1)) * (M[0] - T(1));
+            ps0 = T(0.25) * r[0].p * (M[0] - T(1)) * (M[0] - T(1)) * (T(2) + M[0]);
+        }
+        T Mf = Ms0 + Ms1;
+        T pf = ps0 + ps1;
+        const int K = (Mf > T(0)) ? 1 : 0;
+        T tmp = Mf * a[K] * r[K].rho;
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) {
+            T pflow = pf * nrm[j];
+            flow[j] = tmp * r[K].u[j] + pflow;
+        }
+        flow[DIM] = tmp;
+        flow[DIM + 1] = tmp * H[K];
+    }
 }
 
 // One face: FA[k][f] = F_k * A, the flux from neighbour 1 into neighbour 0
@@ -264,144 +425,28 @@ __global__ void __launch_bounds__(kBlock) k_flux(Geom g, State<T> s, Phys ph) {
 #pragma unroll
     for (int k = 0; k < DIM; ++k) nrm[k] = T(g.fn[(size_t)k * g.F + f]);
     const T A = T(g.fa[f]);
-    const T cp = T(ph.cp), cv = T(ph.cv), rm = T(ph.rm), gam = T(ph.gamma);
     T flow[WS];
     const bool g0 = a0 >= g.N, g1 = a1 >= g.N;
     if (g0 || g1) {
-        // boundary face: physical flux of the ghost state (quirk Q5),
-        // psstConvectiveHLL::calculate :139-201 (same in HLLC/AUSM)
         Side<DIM, T> gs;
         load_side<DIM, T>(s, g.NT, g0 ? a0 : a1, gs);
-        T vn = T(0), vsq = T(0);
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) {
-            vn += gs.u[k] * nrm[k];
-            vsq += gs.u[k] * gs.u[k];
-        }
-        T H = cp * gs.T_ + T(0.5) * vsq;
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) flow[k] = gs.rho * gs.u[k] * vn + gs.p * nrm[k];
-        flow[DIM] = gs.rho * vn;
-        flow[DIM + 1] = gs.rho * vn * H;
+        ghost_flux<DIM, T>(gs, nrm, T(ph.cp), flow);
     } else {
         Side<DIM, T> r[2];
         load_side<DIM, T>(s, g.NT, a0, r[0]);
         load_side<DIM, T>(s, g.NT, a1, r[1]);
-        if (ORDER == 2) reconstruct_dm<DIM, T, LIM>(g, s, a0, a1, r[0], r[1], ph.rm);
-        T vn[2], a[2], H[2], E[2], fi[2][WS], wi[2][WS], M[2];
+        if (ORDER == 2) {
+            T RLR[DIM];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            vn[j] = T(0);
-            T vsq = T(0);
-#pragma unroll
-            for (int k = 0; k < DIM; ++k) {
-                vn[j] += r[j].u[k] * nrm[k];
-                vsq += r[j].u[k] * r[j].u[k];
-            }
-            a[j] = dsqrt(gam * rm * r[j].T_);
-            H[j] = cp * r[j].T_ + T(0.5) * vsq;
-            if (FLUX == PSST_FLUX_AUSM) {
-                M[j] = vn[j] / a[j];
-            } else {
-                E[j] = cv * r[j].T_ + T(0.5) * vsq;
-#pragma unroll
-                for (int k = 0; k < DIM; ++k) {
-                    fi[j][k] = r[j].rho * r[j].u[k] * vn[j] + r[j].p * nrm[k];
-                    wi[j][k] = r[j].rho * r[j].u[k];
-                }
-                wi[j][DIM] = r[j].rho;
-                wi[j][DIM + 1] = r[j].rho * E[j];
-                fi[j][DIM] = r[j].rho * vn[j];
-                fi[j][DIM + 1] = r[j].rho * vn[j] * H[j];
-            }
+            for (int k = 0; k < DIM; ++k) RLR[k] = T(g.cx[(size_t)k * g.NT + a0] - g.cx[(size_t)k * g.NT + a1]);
+            const T *gr = s.grad;
+            const size_t N = (size_t)g.N;
+            reconstruct_dm<DIM, T, LIM>(
+                r[0], r[1], RLR, T(ph.rm),
+                [&](int i, int k) { return gr[(size_t)a0 * GradLayout<DIM>::STRIDE + i * DIM + k]; },
+                [&](int i, int k) { return gr[(size_t)a1 * GradLayout<DIM>::STRIDE + i * DIM + k]; });
         }
-        if (FLUX == PSST_FLUX_HLL || FLUX == PSST_FLUX_HLLC) {
-            // psstConvectiveHLL::calculate :247-286, psstConvectiveHLLC::calculate :531-585
-            T Hr = roe_avg(H[1], H[0], r[1].rho, r[0].rho);
-            T vrn = T(0), vsq = T(0);
-#pragma unroll
-            for (int k = 0; k < DIM; ++k) {
-                T vr = roe_avg(r[1].u[k], r[0].u[k], r[1].rho, r[0].rho);
-                vrn += vr * nrm[k];
-                vsq += vr * vr;
-            }
-            T ar = dsqrt((gam - T(1)) * (Hr - T(0.5) * vsq));
-            T s1 = vn[1] - a[1];
-            T tmp = vrn - ar;
-            if (s1 > tmp) s1 = tmp;
-            T s0 = vn[0] + a[0];
-            tmp = vrn + ar;
-            if (s0 < tmp) s0 = tmp;
-            if (FLUX == PSST_FLUX_HLL) {
-                if (s1 >= T(0)) {
-#pragma unroll
-                    for (int j = 0; j < WS; ++j) flow[j] = fi[1][j];
-                } else if (s1 <= T(0) && s0 >= T(0)) {
-#pragma unroll
-                    for (int j = 0; j < WS; ++j)
-                        flow[j] = (s0 * fi[1][j] - s1 * fi[0][j] + s0 * s1 * (wi[0][j] - wi[1][j])) / (s0 - s1);
-                } else {
-                    // s0 <= 0, or NaN speeds (the reference throws, quirk Q13;
-                    // the NaN propagates into the state and is reported)
-#pragma unroll
-                    for (int j = 0; j < WS; ++j) flow[j] = fi[0][j];
-                }
-            } else {
-                T sh = (r[0].p - r[1].p + r[1].rho * vn[1] * (s1 - vn[1]) - r[0].rho * vn[0] * (s0 - vn[0])) /
-                       (r[1].rho * (s1 - vn[1]) - r[0].rho * (s0 - vn[0]));
-                if (s1 >= T(0)) {
-#pragma unroll
-                    for (int j = 0; j < WS; ++j) flow[j] = fi[1][j];
-                } else if ((s1 <= T(0) && sh >= T(0)) || (sh <= T(0) && s0 >= T(0))) {
-                    const int K = (s1 <= T(0) && sh >= T(0)) ? 1 : 0;
-                    const T sK = K ? s1 : s0;
-                    const T vK = vn[K];
-                    const T tt = sK - sh;
-                    const T PK = r[K].p + r[K].rho * (sK - vK) * (sh - vK);
-#pragma unroll
-                    for (int j = 0; j < DIM; ++j) flow[j] = (sh * (sK * wi[K][j] - fi[K][j]) + sK * PK * nrm[j]) / tt;
-                    flow[DIM] = (sh * (sK * wi[K][DIM] - fi[K][DIM])) / tt;
-                    flow[DIM + 1] = (sh * (sK * wi[K][DIM + 1] - fi[K][DIM + 1]) + sK * PK * sh) / tt;
-                } else {
-#pragma unroll
-                    for (int j = 0; j < WS; ++j) flow[j] = fi[0][j];
-                }
-            }
-        } else {
-            // psstConvectiveAUSM::calculate :804-851
-            T Ms1, ps1, Ms0, ps0;
-            if (M[1] >= T(1)) {
-                Ms1 = M[1];
-                ps1 = r[1].p;
-            } else if (M[1] <= T(-1)) {
-                Ms1 = T(0);
-                ps1 = T(0);
-            } else {
-                Ms1 = T(0.25) * (M[1] + T(1)) * (M[1] + T(1));
-                ps1 = T(0.25) * r[1].p * (M[1] + T(1)) * (M[1] + T(1)) * (T(2) - M[1]);
-            }
-            if (M[0] >= T(1)) {
-                Ms0 = T(0);
-                ps0 = T(0);
-            } else if (M[0] <= T(-1)) {
-                Ms0 = M[0];
-                ps0 = r[0].p;
-            } else {
-                Ms0 = T(-0.25) * (M[0] - T(1)) * (M[0] - T(1));
-                ps0 = T(0.25) * r[0].p * (M[0] - T(1)) * (M[0] - T(1)) * (T(2) + M[0]);
-            }
-            T Mf = Ms0 + Ms1;
-            T pf = ps0 + ps1;
-            const int K = (Mf > T(0)) ? 1 : 0;
-            T tmp = Mf * a[K] * r[K].rho;
-#pragma unroll
-            for (int j = 0; j < DIM; ++j) {
-                T pflow = pf * nrm[j];
-                flow[j] = tmp * r[K].u[j] + pflow;
-            }
-            flow[DIM] = tmp;
-            flow[DIM + 1] = tmp * H[K];
-        }
+        riemann_flux<DIM, T, FLUX>(r, nrm, ph, flow);
     }
 #pragma unroll
     for (int k = 0; k < WS; ++k) s.FA[(size_t)k * g.F + f] = flow[k] * A;
@@ -460,7 +505,7 @@ __device__ __forceinline__ unsigned long long abs_bits(T x) {
 // and column into one of kResSlots slots.
 template <int WS>
 __device__ __forceinline__ void residual_max(unsigned long long (&m)[WS], unsigned long long *res_row) {
-    __shared__ unsigned long long sm[kBlock / 64][WS];
+    __shared__ unsigned long long sm[16][WS];
 #pragma unroll
     for (int j = 0; j < WS; ++j) {
         unsigned long long v = m[j];
@@ -556,10 +601,11 @@ template <int DIM, typename T>
 struct NbOrdered {
     const T *q;
     const T *qn;
+    const int *rank;
     int NT;
     int c;
     __device__ __forceinline__ void operator()(int nb, T *un, T &Tn) const {
-        const T *src = (nb < c) ? qn : q;  // canonical index == update order; ghosts are >= N > c
+        const T *src = (rank[nb] < rank[c]) ? qn : q;  // ghosts rank >= N: stage-start state
 #pragma unroll
         for (int k = 0; k < DIM; ++k) un[k] = src[(size_t)k * NT + nb];
         Tn = src[(size_t)(DIM + 1) * NT + nb];
@@ -590,7 +636,7 @@ __global__ void __launch_bounds__(kBlock) k_update_level(Geom g, State<T> s, Phy
     if (i < count) {
         const int c = cells[i];
         update_cell<DIM, T, PSST_DT_LOCAL1>(g, s, ph, c, alpha, first_stage, last_stage, rmax,
-                                            NbOrdered<DIM, T>{s.q, s.qn, g.NT, c});
+                                            NbOrdered<DIM, T>{s.q, s.qn, g.rank, g.NT, c});
     }
     residual_max<DIM + 2>(rmax, res_row);
 }
@@ -610,7 +656,7 @@ __global__ void __launch_bounds__(1024) k_update_levels_block(Geom g, State<T> s
         for (int i = b + (int)threadIdx.x; i < e; i += (int)blockDim.x) {
             const int c = cells[i];
             update_cell<DIM, T, PSST_DT_LOCAL1>(g, s, ph, c, alpha, first_stage, last_stage, rmax,
-                                                NbOrdered<DIM, T>{s.q, s.qn, g.NT, c});
+                                                NbOrdered<DIM, T>{s.q, s.qn, g.rank, g.NT, c});
         }
         __syncthreads();
     }
@@ -644,9 +690,8 @@ __global__ void __launch_bounds__(1024) k_update_levels_block(Geom g, State<T> s
 // interior part of `qn`.
 // ---------------------------------------------------------------------------
 template <int DIM, typename T>
-__global__ void __launch_bounds__(kBlock) k_bc(Geom g, T *__restrict__ qn, const BCDev *__restrict__ bcs, Phys ph) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= g.G) return;
+__device__ __forceinline__ void bc_apply(const Geom &g, T *__restrict__ qn, const BCDev *__restrict__ bcs,
+                                         const Phys &ph, int i) {
     const int gc = g.N + i;
     const int c = g.gnb[i];
     const BCDev bc = bcs[g.gbc[i]];
@@ -691,6 +736,13 @@ __global__ void __launch_bounds__(kBlock) k_bc(Geom g, T *__restrict__ qn, const
     qn[(DIM + 2) * NT + gc] = p / (Tt * T(ph.rm));
 }
 
+template <int DIM, typename T>
+__global__ void __launch_bounds__(kBlock) k_bc(Geom g, T *__restrict__ qn, const BCDev *__restrict__ bcs, Phys ph) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.G) return;
+    bc_apply<DIM, T>(g, qn, bcs, ph, i);
+}
+
 // Conservative variables from primitives for the interior cells
 // (psstFiniteVolumeGrid.cpp:279-300).
 template <int DIM, typename T>
@@ -709,6 +761,16 @@ __global__ void __launch_bounds__(kBlock) k_set_w(Geom g, const T *__restrict__ 
     T E = T(ph.cv) * q[(DIM + 1) * NT + c] + T(0.5) * vsq;
     W0[(size_t)DIM * g.N + c] = rho;
     W0[(size_t)(DIM + 1) * g.N + c] = rho * E;
+}
+
+__global__ void k_canary(unsigned long long *p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = 0x5053535443414e59ull;  // "PSSTCANY"
+}
+
+// Plain streaming copy of known traffic (counter calibration).
+__global__ void __launch_bounds__(kBlock) k_copy(const double4 *__restrict__ a, double4 *__restrict__ b, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
 }
 
 __global__ void k_fill_u64(unsigned long long *p, size_t n, unsigned long long v) {
@@ -744,6 +806,330 @@ __global__ void k_convert(const A *__restrict__ a, B *__restrict__ b, size_t n) 
     if (i < n) b[i] = (B)a[i];
 }
 
+// ---------------------------------------------------------------------------
+// Fused, tiled kernels.  A workgroup owns a tile: a contiguous range of
+// (Morton-ordered) interior cells.  It stages the tile's cells and their
+// face neighbours ("slots": own cells first, then the halo) in LDS, computes
+// each face of the tile once, and gathers per cell in face-loop order.
+// ---------------------------------------------------------------------------
+struct TileDev {
+    const int *__restrict__ cell_begin;  // [T+1]
+    const int *__restrict__ halo_off;    // [T+1]
+    const int *__restrict__ halo;        // device cell ids
+    const int *__restrict__ face_off;    // [T+1]
+    const unsigned *__restrict__ ends;   // [TF] slot0 | slot1 << 15 | kind << 30
+    const double *__restrict__ fn;       // [DIM][TF]
+    const double *__restrict__ fa;       // [TF]
+    const double *__restrict__ rlr;      // [DIM][TF] x_0 - x_1
+    const double *__restrict__ beta;     // [TF]
+    const int *__restrict__ ghost_off;   // [T+1]
+    const int *__restrict__ ghost;       // ghost device ids
+    const short *__restrict__ visit;     // [Wv][N] local face, face-loop order
+    const short *__restrict__ list;      // [Wl][N] local face, face-list order
+    const double *__restrict__ lsqc;     // [DIM][Wl][N] least-squares weights
+    int TF, Wv, Wl, MS, MF;              // total tile faces, ELL widths, LDS slot/face capacity
+    int n_tiles;
+};
+
+// Tile of this workgroup.  Workgroups are dealt round-robin to the 8 XCDs
+// (each with its own L2); giving XCD x the contiguous tile range
+// [x*per, (x+1)*per) keeps neighbouring tiles -- which share halo cells -- in
+// the same L2.  The grid has 8*per workgroups; those past the last tile exit.
+__device__ __forceinline__ int tile_of_block(int n_tiles) {
+    const int per = (n_tiles + 7) >> 3;
+    return (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+}
+
+__device__ __forceinline__ int end0(unsigned e) { return (int)(e & 0x7fffu); }
+__device__ __forceinline__ int end1(unsigned e) { return (int)((e >> 15) & 0x7fffu); }
+__device__ __forceinline__ int endkind(unsigned e) { return (int)(e >> 30); }
+
+// Stage the tile's slots: q (DIM+3 rows) and the slot -> cell map.
+template <int DIM, typename T>
+__device__ __forceinline__ void stage_slots(const Geom &g, const TileDev &td, const State<T> &s, int cb, int nc, int hb,
+                                            int ns, T *sq, int *scell) {
+    constexpr int NQ = DIM + 3;
+    for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+        const int cell = i < nc ? cb + i : td.halo[hb + i - nc];
+        scell[i] = cell;
+#pragma unroll
+        for (int v = 0; v < NQ; ++v) sq[v * td.MS + i] = s.q[(size_t)v * g.NT + cell];
+    }
+}
+
+// Gradients of one cell (cell-major layout) into registers, wide loads.
+template <int DIM, typename T>
+__device__ __forceinline__ void load_grads(const T *__restrict__ gr, int cell, T (&g)[GradLayout<DIM>::STRIDE]) {
+    constexpr int S = GradLayout<DIM>::STRIDE;
+    if (sizeof(T) == 8) {
+        const double2 *p = reinterpret_cast<const double2 *>(gr + (size_t)cell * S);
+#pragma unroll
+        for (int i = 0; i < S / 2; ++i) {
+            double2 v = p[i];
+            g[2 * i] = (T)v.x;
+            g[2 * i + 1] = (T)v.y;
+        }
+    } else {
+        const float2 *p = reinterpret_cast<const float2 *>(gr + (size_t)cell * S);
+#pragma unroll
+        for (int i = 0; i < S / 2; ++i) {
+            float2 v = p[i];
+            g[2 * i] = (T)v.x;
+            g[2 * i + 1] = (T)v.y;
+        }
+    }
+}
+
+// One RK stage, fused: face fluxes -> per-cell ordered gather -> time step ->
+// update -> primitives -> ghost states of the tile's boundary faces.
+template <int DIM, typename T, int FLUX, int ORDER, int LIM, int LC, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_stage(Geom g, TileDev td, State<T> s, Phys ph, T alpha, int last_stage,
+                                                  unsigned long long *res_row, const BCDev *__restrict__ bcs) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int NQ = DIM + 3, NV = DIM + 2, WS = DIM + 2;
+    T *sq = reinterpret_cast<T *>(smem);
+    T *sfa = sq + NQ * td.MS;
+    int *scell = reinterpret_cast<int *>(sfa + NV * td.MF);
+    const int t = tile_of_block(td.n_tiles);
+    if (t >= td.n_tiles) return;
+    const int cb = td.cell_begin[t], nc = td.cell_begin[t + 1] - cb;
+    const int hb = td.halo_off[t], ns = nc + (td.halo_off[t + 1] - hb);
+    const int fb = td.face_off[t], nf = td.face_off[t + 1] - fb;
+    stage_slots<DIM, T>(g, td, s, cb, nc, hb, ns, sq, scell);
+    __syncthreads();
+    // ---- faces of the tile ----
+    for (int j = threadIdx.x; j < nf; j += blockDim.x) {
+        const unsigned e = td.ends[fb + j];
+        const int l0 = end0(e), l1 = end1(e), kind = endkind(e);
+        T nrm[DIM];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) nrm[k] = T(td.fn[(size_t)k * td.TF + fb + j]);
+        T flow[WS];
+        if (kind != 0) {
+            const int gsl = kind == 1 ? l0 : l1;
+            Side<DIM, T> gs;
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) gs.u[k] = sq[k * td.MS + gsl];
+            gs.p = sq[DIM * td.MS + gsl];
+            gs.T_ = sq[(DIM + 1) * td.MS + gsl];
+            gs.rho = sq[(DIM + 2) * td.MS + gsl];
+            ghost_flux<DIM, T>(gs, nrm, T(ph.cp), flow);
+        } else {
+            Side<DIM, T> r[2];
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) {
+                r[0].u[k] = sq[k * td.MS + l0];
+                r[1].u[k] = sq[k * td.MS + l1];
+            }
+            r[0].p = sq[DIM * td.MS + l0];
+            r[1].p = sq[DIM * td.MS + l1];
+            r[0].T_ = sq[(DIM + 1) * td.MS + l0];
+            r[1].T_ = sq[(DIM + 1) * td.MS + l1];
+            r[0].rho = sq[(DIM + 2) * td.MS + l0];
+            r[1].rho = sq[(DIM + 2) * td.MS + l1];
+            if (ORDER == 2) {
+                T RLR[DIM];
+#pragma unroll
+                for (int k = 0; k < DIM; ++k) RLR[k] = T(td.rlr[(size_t)k * td.TF + fb + j]);
+                T g0[GradLayout<DIM>::STRIDE], g1[GradLayout<DIM>::STRIDE];
+                load_grads<DIM, T>(s.grad, scell[l0], g0);
+                load_grads<DIM, T>(s.grad, scell[l1], g1);
+                reconstruct_dm<DIM, T, LIM>(
+                    r[0], r[1], RLR, T(ph.rm), [&](int i, int k) { return g0[i * DIM + k]; },
+                    [&](int i, int k) { return g1[i * DIM + k]; });
+            }
+            riemann_flux<DIM, T, FLUX>(r, nrm, ph, flow);
+        }
+        const T A = T(td.fa[fb + j]);
+#pragma unroll
+        for (int k = 0; k < WS; ++k) sfa[k * td.MF + j] = flow[k] * A;
+    }
+    __syncthreads();
+    // ---- cells of the tile ----
+    unsigned long long rmax[WS];
+#pragma unroll
+    for (int j = 0; j < WS; ++j) rmax[j] = 0;
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+        const int c = cb + i;
+        T R[WS];
+#pragma unroll
+        for (int j = 0; j < WS; ++j) R[j] = T(0);
+        for (int w = 0; w < td.Wv; ++w) {
+            const int lf = td.visit[(size_t)w * g.N + c];
+            if (lf < 0) break;
+            const bool first = end0(td.ends[fb + lf]) == i;
+#pragma unroll
+            for (int j = 0; j < WS; ++j) {
+                const T v = sfa[j * td.MF + lf];
+                R[j] = first ? R[j] - v : R[j] + v;
+            }
+        }
+        T uc[DIM];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) uc[k] = sq[k * td.MS + i];
+        const T Tc = sq[(DIM + 1) * td.MS + i];
+        T dt;
+        if (LC == PSST_DT_UNIFORM) {
+            dt = T(ph.dt_uniform);
+        } else if (LC == PSST_DT_LOCAL2) {
+            dt = dt_local2<DIM, T>(g, c, uc, Tc, ph);
+        } else {
+            // time step mode 1 with every neighbour at the stage-start state
+            // (Jacobi; psstLocalTimeCalculation.cpp:90-131 otherwise)
+            const T gr = T(ph.gamma) * T(ph.rm);
+            T cc = dsqrt(gr * Tc);
+            T den = T(0), nn = T(0);
+            for (int w = 0; w < td.Wl; ++w) {
+                const int lf = td.list[(size_t)w * g.N + c];
+                if (lf < 0) break;
+                const unsigned e = td.ends[fb + lf];
+                const int nbs = end0(e) == i ? end1(e) : end0(e);
+                T nv = T(0);
+                T cn = (dsqrt(gr * sq[(DIM + 1) * td.MS + nbs]) + cc) * T(0.5);
+#pragma unroll
+                for (int k = 0; k < DIM; ++k) {
+                    T tn = T(td.fn[(size_t)k * td.TF + fb + lf]);
+                    nv += uc[k] * tn;
+                    nn += sq[k * td.MS + nbs] * tn;
+                }
+                nn = (nn + nv) * T(0.5);
+                den += (fabs(nn) + cn) * T(td.fa[fb + lf]);
+            }
+            dt = T(ph.cfl) * T(g.vol[c]) / den;
+        }
+        const T V = T(g.vol[c]);
+        T w[WS];
+#pragma unroll
+        for (int j = 0; j < WS; ++j) {
+            T w0 = s.W0[(size_t)j * g.N + c];
+            w[j] = w0 - dt * alpha * R[j] / V;
+            unsigned long long ab = abs_bits(R[j]);
+            rmax[j] = rmax[j] > ab ? rmax[j] : ab;
+        }
+        if (last_stage)
+#pragma unroll
+            for (int j = 0; j < WS; ++j) s.W0[(size_t)j * g.N + c] = w[j];
+        const T rho = w[DIM];
+        T vsq = T(0);
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            T u = w[k] / rho;
+            s.qn[(size_t)k * g.NT + c] = u;
+            vsq += u * u;
+        }
+        T Tn = ((w[DIM + 1] / rho) - (vsq * T(0.5))) / T(ph.cv);
+        s.qn[(size_t)DIM * g.NT + c] = rho * T(ph.rm) * Tn;
+        s.qn[(size_t)(DIM + 1) * g.NT + c] = Tn;
+        s.qn[(size_t)(DIM + 2) * g.NT + c] = rho;
+    }
+    // ---- ghosts of the tile's boundary faces (psstFiniteVolumeGrid.cpp:351-366) ----
+    __syncthreads();
+    const int gb = td.ghost_off[t], ng = td.ghost_off[t + 1] - gb;
+    for (int i = threadIdx.x; i < ng; i += blockDim.x) bc_apply<DIM, T>(g, s.qn, bcs, ph, td.ghost[gb + i] - g.N);
+    residual_max<WS>(rmax, res_row);
+}
+
+// Green-Gauss gradients on a tile (psstGreenGauss::calculate,
+// psstGradCalculation.cpp:80-177): q of own and halo cells in LDS, each
+// cell's faces in face-loop order.
+template <int DIM, typename T, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_grad_gg_tile(Geom g, TileDev td, State<T> s) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int NV = DIM + 2;
+    T *sq = reinterpret_cast<T *>(smem);
+    const int t = tile_of_block(td.n_tiles);
+    if (t >= td.n_tiles) return;
+    const int cb = td.cell_begin[t], nc = td.cell_begin[t + 1] - cb;
+    const int hb = td.halo_off[t], ns = nc + (td.halo_off[t + 1] - hb);
+    const int fb = td.face_off[t];
+    for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+        const int cell = i < nc ? cb + i : td.halo[hb + i - nc];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sq[v * td.MS + i] = s.q[(size_t)v * g.NT + cell];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+        const int c = cb + i;
+        T acc[NV][DIM];
+#pragma unroll
+        for (int n = 0; n < NV; ++n)
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) acc[n][k] = T(0);
+        const double Vc = g.vol[c];
+        for (int w = 0; w < td.Wv; ++w) {
+            const int lf = td.visit[(size_t)w * g.N + c];
+            if (lf < 0) break;
+            const unsigned e = td.ends[fb + lf];
+            const int l0 = end0(e), l1 = end1(e);
+            const T geom = T(td.fa[fb + lf] / Vc);
+            const T beta = T(td.beta[fb + lf]);
+            const T rbeta = T(1) - beta;
+            T nrm[DIM];
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) nrm[k] = T(td.fn[(size_t)k * td.TF + fb + lf]);
+            const bool first = (l0 == i);
+#pragma unroll
+            for (int n = 0; n < NV; ++n) {
+                T phi = beta * sq[n * td.MS + l0] + rbeta * sq[n * td.MS + l1];
+                T sp = first ? -phi : phi;
+#pragma unroll
+                for (int k = 0; k < DIM; ++k) acc[n][k] += sp * geom * nrm[k];
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < NV; ++n)
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) s.grad[(size_t)c * GradLayout<DIM>::STRIDE + n * DIM + k] = acc[n][k];
+    }
+}
+
+// Least-squares gradients on a tile (psstLeastSquare::calculate,
+// psstGradCalculation.cpp:268-368), face-list order.
+template <int DIM, typename T, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_grad_lsq_tile(Geom g, TileDev td, State<T> s) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int NV = DIM + 2;
+    T *sq = reinterpret_cast<T *>(smem);
+    const int t = tile_of_block(td.n_tiles);
+    if (t >= td.n_tiles) return;
+    const int cb = td.cell_begin[t], nc = td.cell_begin[t + 1] - cb;
+    const int hb = td.halo_off[t], ns = nc + (td.halo_off[t + 1] - hb);
+    const int fb = td.face_off[t];
+    for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+        const int cell = i < nc ? cb + i : td.halo[hb + i - nc];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sq[v * td.MS + i] = s.q[(size_t)v * g.NT + cell];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+        const int c = cb + i;
+        T acc[NV][DIM];
+#pragma unroll
+        for (int n = 0; n < NV; ++n)
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) acc[n][k] = T(0);
+        for (int w = 0; w < td.Wl; ++w) {
+            const int lf = td.list[(size_t)w * g.N + c];
+            if (lf < 0) break;
+            const unsigned e = td.ends[fb + lf];
+            const int nbs = end0(e) == i ? end1(e) : end0(e);
+            T wk[DIM];
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) wk[k] = T(td.lsqc[((size_t)k * td.Wl + w) * g.N + c]);
+#pragma unroll
+            for (int n = 0; n < NV; ++n) {
+                T d = (sq[n * td.MS + nbs] - sq[n * td.MS + i]);
+#pragma unroll
+                for (int k = 0; k < DIM; ++k) acc[n][k] += wk[k] * d;
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < NV; ++n)
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) s.grad[(size_t)c * GradLayout<DIM>::STRIDE + n * DIM + k] = acc[n][k];
+    }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -771,6 +1157,13 @@ struct psst_ctx {
     int max_level = 0;
     // residual normalisation for the tolerance test
     std::vector<double> res0;
+    // device numbering: d2c[d] = canonical index of device cell d (NT entries)
+    std::vector<int32_t> d2c;
+    // fused tiled path
+    bool fused = false;
+    TileDev td{};
+    int n_tiles = 0, tile_threads = 256;
+    size_t shm_stage = 0, shm_grad = 0;
     // timing
     bool timing = false;
     struct KStat {
@@ -870,13 +1263,113 @@ State<T> state(psst_ctx *c) {
 }
 
 // ---- dispatch of the templated kernels ----
+template <typename K>
+void set_shm(K kernel, size_t bytes) {
+    static std::map<const void *, size_t> done;  // largest size already allowed per kernel
+    if (bytes <= 64 * 1024) return;
+    auto it = done.find((const void *)kernel);
+    if (it != done.end() && it->second >= bytes) return;
+    hip_check(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes),
+              "hipFuncSetAttribute");
+    done[(const void *)kernel] = bytes;
+}
+
+template <typename K, typename... Args>
+void launch_shm(psst_ctx *c, const char *name, K kernel, dim3 grid, dim3 block, size_t shm, Args... args) {
+    if (grid.x == 0) return;
+    set_shm(kernel, shm);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        if (c->pending.size() > 4096) flush_timing(c);
+        e0 = get_event(c);
+        e1 = get_event(c);
+        hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
+    }
+    hipLaunchKernelGGL(kernel, grid, block, shm, c->stream, args...);
+    hip_check(hipGetLastError(), name);
+    if (c->timing) {
+        hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
+        c->pending.push_back({name, {e0, e1}});
+    }
+}
+
 template <int DIM, typename T>
 void run_grad(psst_ctx *c) {
     State<T> st = state<T>(c);
+    if (c->fused) {
+        const bool big = c->tile_threads == 512;
+        dim3 gr(8 * ((c->n_tiles + 7) / 8)), bl(c->tile_threads);
+        if (c->s.grad == PSST_GRAD_GREEN_GAUSS) {
+            if (big) launch_shm(c, "grad_gg_tile", k_grad_gg_tile<DIM, T, 512>, gr, bl, c->shm_grad, c->geom, c->td, st);
+            else launch_shm(c, "grad_gg_tile", k_grad_gg_tile<DIM, T, 256>, gr, bl, c->shm_grad, c->geom, c->td, st);
+        } else {
+            if (big) launch_shm(c, "grad_lsq_tile", k_grad_lsq_tile<DIM, T, 512>, gr, bl, c->shm_grad, c->geom, c->td, st);
+            else launch_shm(c, "grad_lsq_tile", k_grad_lsq_tile<DIM, T, 256>, gr, bl, c->shm_grad, c->geom, c->td, st);
+        }
+        return;
+    }
     if (c->s.grad == PSST_GRAD_GREEN_GAUSS)
         launch(c, "grad_gg", k_grad_gg<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st);
     else
         launch(c, "grad_lsq", k_grad_lsq<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st);
+}
+
+template <int DIM, typename T, int FLUX, int ORDER, int LIM, int NTHR>
+void run_stage_lc_t(psst_ctx *c, int k, unsigned long long *res_row) {
+    State<T> st = state<T>(c);
+    const T alpha = T(c->s.rk_coeff[k]);
+    const int last = (k == c->s.rk_levels - 1);
+    const char *nm = "stage";
+    dim3 gr(8 * ((c->n_tiles + 7) / 8)), bl(NTHR);
+    switch (c->s.time_mode) {
+    case PSST_DT_UNIFORM:
+        launch_shm(c, nm, k_stage<DIM, T, FLUX, ORDER, LIM, PSST_DT_UNIFORM, NTHR>, gr, bl, c->shm_stage, c->geom,
+                   c->td, st, c->ph, alpha, last, res_row, (const BCDev *)c->bc);
+        break;
+    case PSST_DT_LOCAL1:
+        launch_shm(c, nm, k_stage<DIM, T, FLUX, ORDER, LIM, PSST_DT_LOCAL1, NTHR>, gr, bl, c->shm_stage, c->geom,
+                   c->td, st, c->ph, alpha, last, res_row, (const BCDev *)c->bc);
+        break;
+    default:
+        launch_shm(c, nm, k_stage<DIM, T, FLUX, ORDER, LIM, PSST_DT_LOCAL2, NTHR>, gr, bl, c->shm_stage, c->geom,
+                   c->td, st, c->ph, alpha, last, res_row, (const BCDev *)c->bc);
+        break;
+    }
+}
+
+template <int DIM, typename T, int FLUX, int ORDER, int LIM>
+void run_stage_lc(psst_ctx *c, int k, unsigned long long *res_row) {
+    if (c->tile_threads == 512) run_stage_lc_t<DIM, T, FLUX, ORDER, LIM, 512>(c, k, res_row);
+    else run_stage_lc_t<DIM, T, FLUX, ORDER, LIM, 256>(c, k, res_row);
+}
+
+template <int DIM, typename T, int FLUX, int ORDER>
+void run_stage_lim(psst_ctx *c, int k, unsigned long long *res_row) {
+    if (ORDER == 1) return run_stage_lc<DIM, T, FLUX, 1, PSST_LIMITER_VAN_LEER>(c, k, res_row);
+    switch (c->s.limiter) {
+    case PSST_LIMITER_VAN_ALBADA:
+        return run_stage_lc<DIM, T, FLUX, 2, PSST_LIMITER_VAN_ALBADA>(c, k, res_row);
+    case PSST_LIMITER_MINMOD:
+        return run_stage_lc<DIM, T, FLUX, 2, PSST_LIMITER_MINMOD>(c, k, res_row);
+    default:
+        return run_stage_lc<DIM, T, FLUX, 2, PSST_LIMITER_VAN_LEER>(c, k, res_row);
+    }
+}
+
+template <int DIM, typename T>
+void run_stage(psst_ctx *c, int k, unsigned long long *res_row) {
+    const bool o2 = c->s.order == 2;
+    switch (c->s.flux) {
+    case PSST_FLUX_AUSM:
+        return o2 ? run_stage_lim<DIM, T, PSST_FLUX_AUSM, 2>(c, k, res_row)
+                  : run_stage_lim<DIM, T, PSST_FLUX_AUSM, 1>(c, k, res_row);
+    case PSST_FLUX_HLL:
+        return o2 ? run_stage_lim<DIM, T, PSST_FLUX_HLL, 2>(c, k, res_row)
+                  : run_stage_lim<DIM, T, PSST_FLUX_HLL, 1>(c, k, res_row);
+    default:
+        return o2 ? run_stage_lim<DIM, T, PSST_FLUX_HLLC, 2>(c, k, res_row)
+                  : run_stage_lim<DIM, T, PSST_FLUX_HLLC, 1>(c, k, res_row);
+    }
 }
 
 template <int DIM, typename T, int FLUX, int ORDER>
@@ -954,6 +1447,13 @@ void run_bc(psst_ctx *c) {
 template <int DIM, typename T>
 void run_iteration(psst_ctx *c, unsigned long long *res_row) {
     if (c->s.order == 2) run_grad<DIM, T>(c);
+    if (c->fused) {
+        for (int k = 0; k < c->s.rk_levels; ++k) {
+            run_stage<DIM, T>(c, k, res_row);
+            c->cur ^= 1;
+        }
+        return;
+    }
     for (int k = 0; k < c->s.rk_levels; ++k) {
         run_flux<DIM, T>(c);
         run_update<DIM, T>(c, k, res_row);
@@ -982,6 +1482,36 @@ void ensure_res(psst_ctx *c, int n_iter) {
 }
 
 size_t state_bytes(const psst_ctx *c) { return c->s.precision == 32 ? sizeof(float) : sizeof(double); }
+
+// canonical <-> device numbering of [rows][n] arrays (n = NT or N)
+void to_device_order(const psst_ctx *c, const double *src, double *dst, int rows, int n) {
+    for (int r = 0; r < rows; ++r)
+        for (int d = 0; d < n; ++d) dst[(size_t)r * n + d] = src[(size_t)r * n + c->d2c[(size_t)d]];
+}
+void to_canonical_order(const psst_ctx *c, const double *src, double *dst, int rows, int n) {
+    for (int r = 0; r < rows; ++r)
+        for (int d = 0; d < n; ++d) dst[(size_t)r * n + c->d2c[(size_t)d]] = src[(size_t)r * n + d];
+}
+
+// device buffer (double or float) -> host double, device order
+void download_raw(psst_ctx *c, const void *dev, double *out, size_t cnt) {
+    if (c->s.precision == 64) {
+        hip_check(hipMemcpyAsync(out, dev, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "D2H");
+    } else {
+        std::vector<float> f(cnt);
+        hip_check(hipMemcpyAsync(f.data(), dev, cnt * sizeof(float), hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "D2H");
+        for (size_t i = 0; i < cnt; ++i) out[i] = f[i];
+    }
+}
+
+// device buffer (double or float) [rows][n] -> host double, canonical order
+void download_canonical(psst_ctx *c, const void *dev, double *out, int rows, int n) {
+    std::vector<double> tmp((size_t)rows * n);
+    download_raw(c, dev, tmp.data(), tmp.size());
+    to_canonical_order(c, tmp.data(), out, rows, n);
+}
 
 psst_status set_error(psst_ctx *c, const std::exception &e, psst_status st) {
     if (c) c->err = e.what();
@@ -1028,7 +1558,7 @@ psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, pss
         c->s.bc = nullptr;
         c->ph = Phys{s->gamma, s->rm, s->cp, s->cv, s->cfl, s->uniform_dt, s->venkat_k};
         const int dim = c->dim, N = c->N, G = c->G, F = c->F, NT = c->NT;
-        // ---- host-side validation of the topology (every index in range) ----
+        // ---- validation of the canonical topology (every index in range) ----
         const int32_t *f0 = m->face_cells, *f1 = m->face_cells + F;
         for (int f = 0; f < F; ++f) {
             if (f0[f] < 0 || f0[f] >= NT || f1[f] < 0 || f1[f] >= NT || f0[f] == f1[f])
@@ -1042,7 +1572,7 @@ psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, pss
         }
         c->nnz_visit = m->visit_offsets[N];
         c->nnz_list = m->list_offsets[N];
-        std::vector<int> lnb((size_t)c->nnz_list);
+        std::vector<int> lnb_c((size_t)c->nnz_list);
         for (int i = 0; i < N; ++i) {
             for (int k = m->visit_offsets[i]; k < m->visit_offsets[i + 1]; ++k) {
                 int f = m->visit_faces[k];
@@ -1051,15 +1581,13 @@ psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, pss
             for (int k = m->list_offsets[i]; k < m->list_offsets[i + 1]; ++k) {
                 int f = m->list_faces[k];
                 if (f < 0 || f >= F || (f0[f] != i && f1[f] != i)) throw std::invalid_argument("face list corrupt");
-                lnb[(size_t)k] = (f0[f] == i) ? f1[f] : f0[f];
+                lnb_c[(size_t)k] = (f0[f] == i) ? f1[f] : f0[f];
             }
         }
-        std::vector<int> gnb((size_t)std::max(G, 1));
         for (int i = 0; i < G; ++i) {
             int f = m->ghost_face[i];
             if (f < 0 || f >= F || (f0[f] != N + i && f1[f] != N + i)) throw std::invalid_argument("ghost_face corrupt");
             if (m->ghost_bc[i] < 0 || m->ghost_bc[i] >= s->n_bc) throw std::invalid_argument("ghost_bc out of range");
-            gnb[(size_t)i] = (f0[f] == N + i) ? f1[f] : f0[f];
             const psst_bc &b = s->bc[m->ghost_bc[i]];
             int dk = b.dynamic_kind, tk = b.thermal_kind;
             if (dk != PSST_BC_SS_INLET && dk != PSST_BC_SS_OUTLET && dk != PSST_BC_SLIP_WALL && dk != PSST_BC_OUTLET)
@@ -1067,32 +1595,72 @@ psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, pss
             if (tk != PSST_BCT_WALL_T && tk != PSST_BCT_WALL_Q && tk != PSST_BCT_INLET && tk != PSST_BCT_OUTLET)
                 throw std::invalid_argument("unsupported thermal boundary kind");
         }
-        // ---- derived host tables ----
-        // projected areas for time step mode 2, in face-list order
-        // (psstLocalTimeCalculation.cpp:192-202)
-        std::vector<double> S(3 * (size_t)N, 0.0);
-        for (int i = 0; i < N; ++i) {
-            double sk[3] = {0, 0, 0};
-            for (int k = m->list_offsets[i]; k < m->list_offsets[i + 1]; ++k) {
-                int f = m->list_faces[k];
-                for (int d = 0; d < dim; ++d) sk[d] += m->face_area[f] * std::abs(m->face_normal[(size_t)d * F + f]);
-            }
-            for (int d = 0; d < 3; ++d) S[(size_t)d * N + i] = sk[d];
+        // ---- device numbering: interior cells along a Morton curve, ghosts unchanged ----
+        std::vector<int32_t> d2c_int = psst::morton_order(dim, N, NT, m->cell_centre);
+        c->d2c.resize((size_t)NT);
+        std::vector<int32_t> c2d((size_t)NT);
+        for (int d = 0; d < N; ++d) c->d2c[(size_t)d] = d2c_int[(size_t)d];
+        for (int d = N; d < NT; ++d) c->d2c[(size_t)d] = d;
+        for (int d = 0; d < NT; ++d) c2d[(size_t)c->d2c[(size_t)d]] = d;
+        const std::vector<int32_t> &d2c = c->d2c;
+        std::vector<int32_t> f0d((size_t)F), f1d((size_t)F);
+        for (int f = 0; f < F; ++f) {
+            f0d[(size_t)f] = c2d[(size_t)f0[f]];
+            f1d[(size_t)f] = c2d[(size_t)f1[f]];
         }
-        // least-squares weights w*theta_k per list entry
-        // (psstGradCalculation.cpp:309-356; 3D: textbook QR weights)
+        std::vector<int32_t> voff((size_t)N + 1), vface((size_t)c->nnz_visit), loff((size_t)N + 1),
+            lface((size_t)c->nnz_list), lnb((size_t)c->nnz_list), lsrc((size_t)c->nnz_list);
+        voff[0] = loff[0] = 0;
+        for (int d = 0; d < N; ++d) {
+            const int cc = d2c[(size_t)d];
+            int o = voff[(size_t)d];
+            for (int k = m->visit_offsets[cc]; k < m->visit_offsets[cc + 1]; ++k) vface[(size_t)o++] = m->visit_faces[k];
+            voff[(size_t)d + 1] = o;
+            o = loff[(size_t)d];
+            for (int k = m->list_offsets[cc]; k < m->list_offsets[cc + 1]; ++k) {
+                lface[(size_t)o] = m->list_faces[k];
+                lnb[(size_t)o] = c2d[(size_t)lnb_c[(size_t)k]];
+                lsrc[(size_t)o] = k;
+                ++o;
+            }
+            loff[(size_t)d + 1] = o;
+        }
+        std::vector<int> gnb((size_t)std::max(G, 1));
+        for (int i = 0; i < G; ++i) {
+            int f = m->ghost_face[i];
+            gnb[(size_t)i] = (f0d[(size_t)f] == N + i) ? f1d[(size_t)f] : f0d[(size_t)f];
+        }
+        std::vector<double> vol_d((size_t)NT), cx_d(3 * (size_t)NT, 0.0), fn3(3 * (size_t)F, 0.0);
+        for (int d = 0; d < NT; ++d) {
+            vol_d[(size_t)d] = m->cell_volume[d2c[(size_t)d]];
+            for (int k = 0; k < dim; ++k) cx_d[(size_t)k * NT + d] = m->cell_centre[(size_t)k * NT + d2c[(size_t)d]];
+        }
+        for (int k = 0; k < dim; ++k) std::memcpy(&fn3[(size_t)k * F], m->face_normal + (size_t)k * F, sizeof(double) * F);
+        // projected areas for time step mode 2, face-list order (psstLocalTimeCalculation.cpp:192-202)
+        std::vector<double> S(3 * (size_t)N, 0.0);
+        for (int d = 0; d < N; ++d) {
+            double sk[3] = {0, 0, 0};
+            for (int k = loff[(size_t)d]; k < loff[(size_t)d + 1]; ++k) {
+                int f = lface[(size_t)k];
+                for (int x = 0; x < dim; ++x) sk[x] += m->face_area[f] * std::abs(m->face_normal[(size_t)x * F + f]);
+            }
+            for (int x = 0; x < 3; ++x) S[(size_t)x * N + d] = sk[x];
+        }
+        // least-squares weights w*theta_k per list entry (psstGradCalculation.cpp:309-356;
+        // 3D: textbook QR weights), computed in canonical terms
         std::vector<double> lsqc(3 * (size_t)std::max(c->nnz_list, 1), 0.0);
         if (s->grad == PSST_GRAD_LEAST_SQUARES && m->lsq_r) {
-            for (int i = 0; i < N; ++i) {
+            for (int d = 0; d < N; ++d) {
+                const int i = d2c[(size_t)d];
                 const double ra = m->lsq_r[(size_t)0 * N + i], rb = m->lsq_r[(size_t)1 * N + i],
                              rd = m->lsq_r[(size_t)2 * N + i], re = m->lsq_r[(size_t)4 * N + i],
                              rg = m->lsq_r[(size_t)5 * N + i], beta = m->lsq_beta[i];
-                for (int k = m->list_offsets[i]; k < m->list_offsets[i + 1]; ++k) {
-                    int nb = lnb[(size_t)k];
+                for (int k = loff[(size_t)d]; k < loff[(size_t)d + 1]; ++k) {
+                    const int nb = lnb_c[(size_t)lsrc[(size_t)k]];
                     double diff[3] = {0, 0, 0}, w = 0.0, alpha[3] = {0, 0, 0}, theta[3] = {0, 0, 0};
-                    for (int d = 0; d < dim; ++d) {
-                        diff[d] = m->cell_centre[(size_t)d * NT + nb] - m->cell_centre[(size_t)d * NT + i];
-                        w += diff[d] * diff[d];
+                    for (int x = 0; x < dim; ++x) {
+                        diff[x] = m->cell_centre[(size_t)x * NT + nb] - m->cell_centre[(size_t)x * NT + i];
+                        w += diff[x] * diff[x];
                     }
                     double ws = 1.0 / w;
                     if (dim == 2) {
@@ -1108,20 +1676,41 @@ psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, pss
                         theta[1] = alpha[1] - re / rd * alpha[2];
                         theta[2] = alpha[2];
                     }
-                    for (int d = 0; d < 3; ++d) lsqc[(size_t)d * c->nnz_list + k] = ws * theta[d];
+                    for (int x = 0; x < 3; ++x) lsqc[(size_t)x * c->nnz_list + k] = ws * theta[x];
                 }
+            }
+        }
+        hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
+        {
+            // a kernel of this library must demonstrably run: a HIP runtime
+            // other than the one it was built for (e.g. one bundled with
+            // another framework and loaded first under the same soname) can
+            // leave launches without effect and without an error
+            unsigned long long *probe = dev_alloc<unsigned long long>(c.get(), 1);
+            unsigned long long zero = 0, got = 0;
+            hip_check(hipMemcpy(probe, &zero, sizeof(zero), hipMemcpyHostToDevice), "hipMemcpy");
+            hipLaunchKernelGGL(k_canary, dim3(1), dim3(64), 0, c->stream, probe);
+            hip_check(hipGetLastError(), "canary launch");
+            hip_check(hipStreamSynchronize(c->stream), "canary");
+            hip_check(hipMemcpy(&got, probe, sizeof(got), hipMemcpyDeviceToHost), "hipMemcpy");
+            if (got != 0x5053535443414e59ull) {
+                int rv = 0;
+                (void)hipRuntimeGetVersion(&rv);
+                throw HipError("gfx950 kernels of libpsst did not run (HIP runtime version " + std::to_string(rv) +
+                               " in this process): load libpsst before any library that bundles its own HIP "
+                               "runtime, or use the system ROCm runtime");
             }
         }
         // reference-ordered schedule for time step mode 1: level of a cell =
         // 1 + max level of its interior face-list neighbours earlier in the
-        // update loop (canonical index order)
+        // update loop (canonical index order); stored as device ids
         if (s->time_mode == PSST_DT_LOCAL1 && s->dt_ordering == 0) {
             std::vector<int> lev((size_t)N, 0);
             int maxl = 0;
             for (int i = 0; i < N; ++i) {
                 int l = 0;
                 for (int k = m->list_offsets[i]; k < m->list_offsets[i + 1]; ++k) {
-                    int nb = lnb[(size_t)k];
+                    int nb = lnb_c[(size_t)k];
                     if (nb < i) l = std::max(l, lev[(size_t)nb] + 1);
                 }
                 lev[(size_t)i] = l;
@@ -1132,14 +1721,11 @@ psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, pss
             for (int l = 0; l <= maxl; ++l) c->level_off[(size_t)l + 1] += c->level_off[(size_t)l];
             std::vector<int> cells((size_t)N), fill(c->level_off.begin(), c->level_off.end() - 1);
             int widest = 0;
-            for (int i = 0; i < N; ++i) cells[(size_t)fill[(size_t)lev[(size_t)i]]++] = i;
+            for (int i = 0; i < N; ++i) cells[(size_t)fill[(size_t)lev[(size_t)i]]++] = c2d[(size_t)i];
             for (int l = 0; l <= maxl; ++l) widest = std::max(widest, c->level_off[(size_t)l + 1] - c->level_off[(size_t)l]);
             c->max_level = widest;
-            hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
             c->level_cells = dev_upload(c.get(), cells.data(), cells.size());
             c->level_off_dev = dev_upload(c.get(), c->level_off.data(), c->level_off.size());
-        } else {
-            hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
         }
         // ---- device geometry ----
         Geom &g = c->geom;
@@ -1147,28 +1733,27 @@ psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, pss
         g.G = G;
         g.NT = NT;
         g.F = F;
-        std::vector<double> cx3(3 * (size_t)NT, 0.0), fn3(3 * (size_t)F, 0.0);
-        for (int d = 0; d < dim; ++d) {
-            std::memcpy(&cx3[(size_t)d * NT], m->cell_centre + (size_t)d * NT, sizeof(double) * NT);
-            std::memcpy(&fn3[(size_t)d * F], m->face_normal + (size_t)d * F, sizeof(double) * F);
-        }
-        g.vol = dev_upload(c.get(), m->cell_volume, (size_t)NT);
-        g.cx = dev_upload(c.get(), cx3.data(), cx3.size());
+        g.vol = dev_upload(c.get(), vol_d.data(), vol_d.size());
+        g.cx = dev_upload(c.get(), cx_d.data(), cx_d.size());
         g.fn = dev_upload(c.get(), fn3.data(), fn3.size());
         g.fa = dev_upload(c.get(), m->face_area, (size_t)F);
         g.beta = dev_upload(c.get(), m->face_beta, (size_t)F);
-        g.f0 = dev_upload(c.get(), f0, (size_t)F);
-        g.f1 = dev_upload(c.get(), f1, (size_t)F);
-        g.voff = dev_upload(c.get(), m->visit_offsets, (size_t)N + 1);
-        g.vface = dev_upload(c.get(), m->visit_faces, (size_t)c->nnz_visit);
-        g.loff = dev_upload(c.get(), m->list_offsets, (size_t)N + 1);
-        g.lface = dev_upload(c.get(), m->list_faces, (size_t)c->nnz_list);
+        g.f0 = dev_upload(c.get(), f0d.data(), (size_t)F);
+        g.f1 = dev_upload(c.get(), f1d.data(), (size_t)F);
+        g.voff = dev_upload(c.get(), voff.data(), voff.size());
+        g.vface = dev_upload(c.get(), vface.data(), vface.size());
+        g.loff = dev_upload(c.get(), loff.data(), loff.size());
+        g.lface = dev_upload(c.get(), lface.data(), lface.size());
         g.lnb = dev_upload(c.get(), lnb.data(), lnb.size());
         g.lsqc = dev_upload(c.get(), lsqc.data(), lsqc.size());
         g.S = dev_upload(c.get(), S.data(), S.size());
         g.gface = dev_upload(c.get(), m->ghost_face, (size_t)G);
         g.gnb = dev_upload(c.get(), gnb.data(), (size_t)G);
         g.gbc = dev_upload(c.get(), m->ghost_bc, (size_t)G);
+        {
+            std::vector<int32_t> rank(d2c.begin(), d2c.end());
+            g.rank = dev_upload(c.get(), rank.data(), rank.size());
+        }
         std::vector<BCDev> bcd((size_t)s->n_bc);
         for (int i = 0; i < s->n_bc; ++i) {
             bcd[(size_t)i].dyn = s->bc[i].dynamic_kind;
@@ -1178,20 +1763,82 @@ psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, pss
             bcd[(size_t)i].T = s->bc[i].temperature;
         }
         c->bc = dev_upload(c.get(), bcd.data(), bcd.size());
-        // ---- state ----
         const size_t sb = state_bytes(c.get());
+        // ---- tile plan of the fused kernels ----
+        const bool ordered_dt1 = (s->time_mode == PSST_DT_LOCAL1 && s->dt_ordering == 0);
+        if (s->fused && !ordered_dt1) {
+            const int tile = s->tile_cells > 0 ? s->tile_cells : 256;
+            if (tile < 32 || tile > 4096) throw std::invalid_argument("tile_cells must be 32..4096");
+            if (s->tile_threads != 0 && s->tile_threads != 256 && s->tile_threads != 512)
+                throw std::invalid_argument("tile_threads must be 256 or 512");
+            c->tile_threads = s->tile_threads ? s->tile_threads : 256;
+            psst::TilePlan tp = psst::plan_tiles(N, G, F, tile, f0d.data(), f1d.data(), voff.data(), vface.data(),
+                                                 loff.data(), lface.data());
+            const int NQ = dim + 3, NV = dim + 2;
+            const size_t shm_stage =
+                ((size_t)NQ * tp.max_slots + (size_t)NV * tp.max_faces) * sb + (size_t)tp.max_slots * sizeof(int);
+            const size_t shm_grad = (size_t)NV * tp.max_slots * sb;
+            if (shm_stage <= 160 * 1024) {
+                const int TF = (int)tp.face_id.size();
+                std::vector<double> tfn((size_t)dim * TF), tfa((size_t)TF), trlr((size_t)dim * TF), tbeta((size_t)TF);
+                for (int j = 0; j < TF; ++j) {
+                    const int f = tp.face_id[(size_t)j];
+                    for (int k = 0; k < dim; ++k) {
+                        tfn[(size_t)k * TF + j] = m->face_normal[(size_t)k * F + f];
+                        trlr[(size_t)k * TF + j] =
+                            m->cell_centre[(size_t)k * NT + f0[f]] - m->cell_centre[(size_t)k * NT + f1[f]];
+                    }
+                    tfa[(size_t)j] = m->face_area[f];
+                    tbeta[(size_t)j] = m->face_beta[f];
+                }
+                std::vector<double> tlsq((size_t)dim * std::max(tp.W_list, 1) * N, 0.0);
+                if (s->grad == PSST_GRAD_LEAST_SQUARES)
+                    for (int d = 0; d < N; ++d)
+                        for (int k = loff[(size_t)d], w = 0; k < loff[(size_t)d + 1]; ++k, ++w)
+                            for (int x = 0; x < dim; ++x)
+                                tlsq[((size_t)x * tp.W_list + w) * N + d] = lsqc[(size_t)x * c->nnz_list + k];
+                TileDev &td = c->td;
+                td.cell_begin = dev_upload(c.get(), tp.cell_begin.data(), tp.cell_begin.size());
+                td.halo_off = dev_upload(c.get(), tp.halo_off.data(), tp.halo_off.size());
+                td.halo = dev_upload(c.get(), tp.halo.data(), tp.halo.size());
+                td.face_off = dev_upload(c.get(), tp.face_off.data(), tp.face_off.size());
+                td.ends = dev_upload(c.get(), tp.face_ends.data(), tp.face_ends.size());
+                td.fn = dev_upload(c.get(), tfn.data(), tfn.size());
+                td.fa = dev_upload(c.get(), tfa.data(), tfa.size());
+                td.rlr = dev_upload(c.get(), trlr.data(), trlr.size());
+                td.beta = dev_upload(c.get(), tbeta.data(), tbeta.size());
+                td.ghost_off = dev_upload(c.get(), tp.ghost_off.data(), tp.ghost_off.size());
+                td.ghost = dev_upload(c.get(), tp.ghost.data(), tp.ghost.size());
+                td.visit = reinterpret_cast<const short *>(dev_upload(c.get(), tp.visit_ell.data(), tp.visit_ell.size()));
+                td.list = reinterpret_cast<const short *>(dev_upload(c.get(), tp.list_ell.data(), tp.list_ell.size()));
+                td.lsqc = dev_upload(c.get(), tlsq.data(), tlsq.size());
+                td.TF = TF;
+                td.Wv = tp.W_visit;
+                td.Wl = tp.W_list;
+                td.MS = tp.max_slots;
+                td.MF = tp.max_faces;
+                td.n_tiles = tp.n_tiles;
+                c->n_tiles = tp.n_tiles;
+                c->shm_stage = shm_stage;
+                c->shm_grad = shm_grad;
+                c->fused = true;
+            }
+        }
+        // ---- state ----
         for (int i = 0; i < 2; ++i) {
             hip_check(hipMalloc(&c->q[i], sb * (size_t)(dim + 3) * NT), "hipMalloc");
             c->allocs.push_back(c->q[i]);
         }
         hip_check(hipMalloc(&c->W0, sb * (size_t)(dim + 2) * N), "hipMalloc");
         c->allocs.push_back(c->W0);
-        hip_check(hipMalloc(&c->grad, sb * (size_t)(dim + 2) * dim * N), "hipMalloc");
+        hip_check(hipMalloc(&c->grad, sb * (size_t)(dim == 2 ? GradLayout<2>::STRIDE : GradLayout<3>::STRIDE) * N),
+                  "hipMalloc");
         c->allocs.push_back(c->grad);
         hip_check(hipMalloc(&c->FA, sb * (size_t)(dim + 2) * F), "hipMalloc");
         c->allocs.push_back(c->FA);
         ensure_res(c.get(), 64);
-        c->kernel_names = {"grad_gg", "grad_lsq", "flux", "update", "update_level", "update_levels_block", "bc"};
+        c->kernel_names = {"grad_gg",      "grad_lsq",   "flux", "update", "update_level", "update_levels_block",
+                           "bc",           "stage",      "grad_gg_tile", "grad_lsq_tile"};
     } catch (const std::exception &e) {
         g_create_error = e.what();
         for (void *p : c->allocs) (void)hipFree(p);
@@ -1206,13 +1853,15 @@ psst_status psst_set_state(psst_ctx *c, const double *prim) {
     if (!c || !prim) return PSST_BAD_ARG;
     try {
         const size_t n = (size_t)(c->dim + 3) * c->NT;
+        std::vector<double> dev_prim(n);
+        to_device_order(c, prim, dev_prim.data(), c->dim + 3, c->NT);
         if (c->s.precision == 64) {
-            hip_check(hipMemcpy(c->q[0], prim, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
-            hip_check(hipMemcpy(c->q[1], prim, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+            hip_check(hipMemcpy(c->q[0], dev_prim.data(), n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+            hip_check(hipMemcpy(c->q[1], dev_prim.data(), n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
         } else {
             double *tmp = nullptr;
             hip_check(hipMalloc(&tmp, n * sizeof(double)), "hipMalloc");
-            hip_check(hipMemcpy(tmp, prim, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+            hip_check(hipMemcpy(tmp, dev_prim.data(), n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
             hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks((long)n)), dim3(kBlock), 0, c->stream, tmp,
                                (float *)c->q[0], n);
             hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks((long)n)), dim3(kBlock), 0, c->stream, tmp,
@@ -1310,16 +1959,7 @@ psst_status psst_run(psst_ctx *c, int32_t n_iter, double tol, double *res_hist, 
 psst_status psst_get_state(psst_ctx *c, double *prim) {
     if (!c || !prim) return PSST_BAD_ARG;
     try {
-        const size_t n = (size_t)(c->dim + 3) * c->NT;
-        if (c->s.precision == 64) {
-            hip_check(hipMemcpyAsync(prim, c->q[c->cur], n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
-            hip_check(hipStreamSynchronize(c->stream), "D2H");
-        } else {
-            std::vector<float> tmp(n);
-            hip_check(hipMemcpyAsync(tmp.data(), c->q[c->cur], n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "D2H");
-            hip_check(hipStreamSynchronize(c->stream), "D2H");
-            for (size_t i = 0; i < n; ++i) prim[i] = tmp[i];
-        }
+        download_canonical(c, c->q[c->cur], prim, c->dim + 3, c->NT);
         return PSST_OK;
     } catch (const std::exception &e) {
         return set_error(c, e, PSST_HIP_ERROR);
@@ -1329,16 +1969,7 @@ psst_status psst_get_state(psst_ctx *c, double *prim) {
 psst_status psst_get_conservative(psst_ctx *c, double *w) {
     if (!c || !w) return PSST_BAD_ARG;
     try {
-        const size_t n = (size_t)(c->dim + 2) * c->N;
-        if (c->s.precision == 64) {
-            hip_check(hipMemcpyAsync(w, c->W0, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
-            hip_check(hipStreamSynchronize(c->stream), "D2H");
-        } else {
-            std::vector<float> tmp(n);
-            hip_check(hipMemcpyAsync(tmp.data(), c->W0, n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "D2H");
-            hip_check(hipStreamSynchronize(c->stream), "D2H");
-            for (size_t i = 0; i < n; ++i) w[i] = tmp[i];
-        }
+        download_canonical(c, c->W0, w, c->dim + 2, c->N);
         return PSST_OK;
     } catch (const std::exception &e) {
         return set_error(c, e, PSST_HIP_ERROR);
@@ -1353,15 +1984,14 @@ psst_status psst_debug_gradients(psst_ctx *c, double *grads) {
         } else {
             c->s.precision == 64 ? run_grad<3, double>(c) : run_grad<3, float>(c);
         }
-        const size_t n = (size_t)(c->dim + 2) * c->dim * c->N;
-        if (c->s.precision == 64) {
-            hip_check(hipMemcpyAsync(grads, c->grad, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
-            hip_check(hipStreamSynchronize(c->stream), "D2H");
-        } else {
-            std::vector<float> tmp(n);
-            hip_check(hipMemcpyAsync(tmp.data(), c->grad, n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "D2H");
-            hip_check(hipStreamSynchronize(c->stream), "D2H");
-            for (size_t i = 0; i < n; ++i) grads[i] = tmp[i];
+        {
+            const int nc = (c->dim + 2) * c->dim;
+            const int stride = c->dim == 2 ? GradLayout<2>::STRIDE : GradLayout<3>::STRIDE;
+            std::vector<double> aos((size_t)stride * c->N), soa((size_t)nc * c->N);
+            download_raw(c, c->grad, aos.data(), aos.size());
+            for (int d = 0; d < c->N; ++d)
+                for (int v = 0; v < nc; ++v) soa[(size_t)v * c->N + d] = aos[(size_t)d * stride + v];
+            to_canonical_order(c, soa.data(), grads, nc, c->N);
         }
         if (c->timing) flush_timing(c);
         return PSST_OK;
@@ -1398,9 +2028,11 @@ psst_status psst_debug_residual(psst_ctx *c, double *residual) {
             }
         }
         hip_check(hipGetLastError(), "gather");
-        hip_check(hipMemcpyAsync(residual, out, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+        std::vector<double> tmp(n);
+        hip_check(hipMemcpyAsync(tmp.data(), out, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
         hip_check(hipStreamSynchronize(c->stream), "D2H");
         hip_check(hipFree(out), "hipFree");
+        to_canonical_order(c, tmp.data(), residual, c->dim + 2, c->N);
         if (c->timing) flush_timing(c);
         return PSST_OK;
     } catch (const std::exception &e) {
@@ -1443,6 +2075,43 @@ psst_status psst_kernel_names(psst_ctx *c, char *buf, int32_t len) {
     if ((int32_t)s.size() + 1 > len) return PSST_BAD_ARG;
     std::memcpy(buf, s.c_str(), s.size() + 1);
     return PSST_OK;
+}
+
+psst_status psst_copy_calibration(int64_t bytes, int32_t reps, double *mean_ms) {
+    if (bytes <= 0 || reps <= 0) return PSST_BAD_ARG;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    psst_status rc = PSST_OK;
+    try {
+        const size_t n = (size_t)bytes / sizeof(double4);
+        hip_check(hipMalloc(&a, n * sizeof(double4)), "hipMalloc");
+        hip_check(hipMalloc(&b, n * sizeof(double4)), "hipMalloc");
+        hip_check(hipMemset(a, 0, n * sizeof(double4)), "hipMemset");
+        hip_check(hipStreamCreate(&st), "hipStreamCreate");
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        const unsigned grid = 256 * 8 * 4;
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, st, (const double4 *)a, (double4 *)b, n);
+        hip_check(hipEventRecord(e0, st), "hipEventRecord");
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, st, (const double4 *)a, (double4 *)b, n);
+        hip_check(hipGetLastError(), "k_copy");
+        hip_check(hipEventRecord(e1, st), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        if (mean_ms) *mean_ms = ms / reps;
+    } catch (const std::exception &e) {
+        g_create_error = e.what();
+        rc = PSST_HIP_ERROR;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
 }
 
 void psst_ctx_destroy(psst_ctx *c) {

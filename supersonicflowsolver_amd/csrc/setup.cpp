@@ -106,7 +106,7 @@ void load_settings(Case &cs, const std::string &prefix) {
     s = psst_settings{};
     s.precision = 64;
     s.dt_ordering = 0;
-    s.fused = 0;
+    s.fused = 1;
     s.venkat_k = 5.0;
     cs.prim.assign((size_t)(dim + 3) * NT, 0.0);
     PrimView q{cs.prim, dim, NT};

@@ -49,16 +49,19 @@ def test_kernel_parity_gradients_and_residual(tmp_path, name):
         assert _first_mismatch(r_gpu, r_cpu) is None, _first_mismatch(r_gpu, r_cpu)
 
 
+@pytest.mark.parametrize("fused", [1, 0])
 @pytest.mark.parametrize("name", GENERATED_CASES + MATRIX_CASES)
-def test_full_run_matches_reference_build(tmp_path, name):
+def test_full_run_matches_reference_build(tmp_path, name, fused):
     """Whole run (fixture iteration count): final state bit-identical to the
     reference build's, residual history bit-identical to the oracle's and equal
-    to the reference's residual file."""
+    to the reference's residual file.  fused=1: tiled stage kernels (the
+    reference-ordered time step mode 1 keeps the unfused sweep); fused=0:
+    face-flux + gather kernels."""
     mesh, prefix, exp, meta = unpack(name, str(tmp_path))
     case = ss.Case.open(mesh, prefix)
     n = meta["iterations"]
-    print("case %s: %d cells, %d iterations" % (name, case.n_cells, n), flush=True)
-    with ss.Solver(case) as s:
+    print("case %s: %d cells, %d iterations, fused=%d" % (name, case.n_cells, n, fused), flush=True)
+    with ss.Solver(case, fused=fused) as s:
         res = s.run(n)
         st = s.state()
     assert _first_mismatch(st, exp["prim"]) is None, _first_mismatch(st, exp["prim"])
@@ -132,3 +135,58 @@ def test_jacobi_time_step_differs_but_converges(tmp_path):
     assert not np.array_equal(st, exp["prim"])
     rel = np.linalg.norm(st - exp["prim"], axis=1) / np.linalg.norm(exp["prim"], axis=1)
     assert rel.max() < 5e-3
+
+
+@pytest.mark.parametrize("numbering", [1, 2])
+@pytest.mark.parametrize("time_mode", [2, 0])
+def test_fused_matches_oracle_on_renumbered_meshes(tmp_path, numbering, time_mode):
+    """RCM and randomly permuted cell numberings (the device renumbers along a
+    Morton curve): fused and unfused paths bit-identical to the oracle."""
+    mesh = str(tmp_path / "m.msh")
+    prefix = str(tmp_path / "c")
+    ss.write_wedge_msh(ss.wedge_params(nx=90, ny=30, jitter=0.1, numbering=numbering, perm_seed=5), mesh)
+    ss.write_case_files(prefix, time_mode=time_mode, iterations=25, flux=3, order=2, grad=1, limiter=2)
+    case = ss.Case.open(mesh, prefix)
+    o = Oracle(mesh, prefix)
+    ref_res = o.run(25)
+    ref = o.state()
+    for fused in (1, 0):
+        with ss.Solver(case, fused=fused) as s:
+            res = s.run(25)
+            assert _first_mismatch(s.state(), ref) is None, (fused, _first_mismatch(s.state(), ref))
+            assert np.array_equal(res, ref_res)
+
+
+@pytest.mark.parametrize("grad", [1, 2])
+def test_jacobi_dt_fused_equals_unfused(tmp_path, grad):
+    """Time step mode 1 with Jacobi neighbours: the fused and unfused kernels
+    agree bit for bit (same arithmetic, different kernel structure)."""
+    mesh, prefix, exp, meta = unpack("c1_lc1", str(tmp_path))
+    case = ss.Case.open(mesh, prefix)
+    out = []
+    for fused in (1, 0):
+        with ss.Solver(case, fused=fused, dt_ordering=1, grad=grad) as s:
+            s.run(40)
+            out.append(s.state())
+    assert _first_mismatch(out[0], out[1]) is None
+
+
+def test_full_size_wedge_one_iteration_matches_oracle(tmp_path):
+    """The benchmark's full-size mesh (3840 x 1280 = 4,915,200 cells, jitter,
+    RCM numbering): one iteration bit-identical to the oracle, fused and
+    unfused."""
+    p = ss.wedge_params(nx=3840, ny=1280, jitter=0.1, jitter_seed=1, numbering=1)
+    prefix = str(tmp_path / "c")
+    ss.write_case_files(prefix, time_mode=2, iterations=1, flux=3, order=2, grad=1, limiter=2)
+    case = ss.Case.wedge(p, prefix)
+    print("full size: %d cells" % case.n_cells, flush=True)
+    raw = case.raw()
+    raw["dim"] = 2
+    o = Oracle(None, prefix, raw=raw)
+    ref_res = o.run(1)
+    ref = o.state()
+    for fused in (1, 0):
+        with ss.Solver(case, fused=fused) as s:
+            res = s.run(1)
+            assert np.array_equal(res, ref_res), fused
+            assert _first_mismatch(s.state(), ref) is None, (fused, _first_mismatch(s.state(), ref))
