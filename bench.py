@@ -166,8 +166,18 @@ def main():
     nbytes = alg_bytes(kind, N, G, F, nnz, 2, b, last_stage=(dom == "stage_last"))
     mean_ms = stats[dom][1]
     achieved = nbytes / (mean_ms * 1e-3) / 1e9
+    # HBM traffic of the dominant kernel: rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs (each scaled by
+    # the counters' reading for a copy kernel of known size), recorded under profiles/ for the default
+    # workload; null for other configurations
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_wedge3840x1280.json")
+    if os.path.exists(tpath) and a.precision == 64 and (a.nx, a.ny, a.flux) == (3840, 1280, "hllc"):
+        with open(tpath) as f:
+            tr = json.load(f).get(dom)
+        if tr:
+            traffic = tr["bytes_per_launch"]
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "alg_bytes_per_launch": int(nbytes), "mean_ms": round(mean_ms, 4)}
     whole = survey_bytes_per_cell_iteration(2, b, N, F)
     out = {

@@ -94,12 +94,73 @@ struct State {
     T *__restrict__ FA;       // [(dim+2)][F]
 };
 
+// Store one cell's gradients (cell-major layout), wide stores.
+template <int DIM, typename T>
+__device__ __forceinline__ void store_grads(T *__restrict__ gr, int cell, const T (&acc)[DIM + 2][DIM]) {
+    constexpr int S = GradLayout<DIM>::STRIDE, NC = GradLayout<DIM>::NC;
+    T flat[S];
+#pragma unroll
+    for (int n = 0; n < DIM + 2; ++n)
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) flat[n * DIM + k] = acc[n][k];
+#pragma unroll
+    for (int i = NC; i < S; ++i) flat[i] = T(0);
+    if (sizeof(T) == 8) {
+        double2 *p = reinterpret_cast<double2 *>(gr + (size_t)cell * S);
+#pragma unroll
+        for (int i = 0; i < S / 2; ++i) p[i] = make_double2((double)flat[2 * i], (double)flat[2 * i + 1]);
+    } else {
+        float2 *p = reinterpret_cast<float2 *>(gr + (size_t)cell * S);
+#pragma unroll
+        for (int i = 0; i < S / 2; ++i) p[i] = make_float2((float)flat[2 * i], (float)flat[2 * i + 1]);
+    }
+}
+
+// Gradients of one cell (cell-major layout) into registers, wide loads.
+template <int DIM, typename T>
+__device__ __forceinline__ void load_grads(const T *__restrict__ gr, int cell, T (&g)[GradLayout<DIM>::STRIDE]) {
+    constexpr int S = GradLayout<DIM>::STRIDE;
+    if (sizeof(T) == 8) {
+        const double2 *p = reinterpret_cast<const double2 *>(gr + (size_t)cell * S);
+#pragma unroll
+        for (int i = 0; i < S / 2; ++i) {
+            double2 v = p[i];
+            g[2 * i] = (T)v.x;
+            g[2 * i + 1] = (T)v.y;
+        }
+    } else {
+        const float2 *p = reinterpret_cast<const float2 *>(gr + (size_t)cell * S);
+#pragma unroll
+        for (int i = 0; i < S / 2; ++i) {
+            float2 v = p[i];
+            g[2 * i] = (T)v.x;
+            g[2 * i + 1] = (T)v.y;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Small device helpers
 // ---------------------------------------------------------------------------
+// Division and square root of the state type: IEEE-correct for double (the
+// parity path); the hardware's approximate forms for float (FP32 mode makes
+// no bitwise claim, and they cost a fraction of the correctly rounded
+// sequences).
 template <typename T>
 __device__ __forceinline__ T dsqrt(T x) {
     return sqrt(x);
+}
+template <>
+__device__ __forceinline__ float dsqrt<float>(float x) {
+    return __builtin_amdgcn_sqrtf(x);
+}
+template <typename T>
+__device__ __forceinline__ T fdiv(T a, T b) {
+    return a / b;
+}
+template <>
+__device__ __forceinline__ float fdiv<float>(float a, float b) {
+    return __fdividef(a, b);
 }
 
 // psstScalarLimiters.cpp:17-93; eps is absolute (physical units).
@@ -109,18 +170,18 @@ __device__ __forceinline__ T limiter(T num, T den) {
     if ((num > T(0) && den < T(0)) || (num < T(0) && den > T(0))) return T(0);
     if (LIM == PSST_LIMITER_VAN_ALBADA) {
         if (fabs(den) < eps) return T(1);
-        T r = num / den;
-        return (r * r + r) / (r * r + T(1));
+        T r = fdiv(num, den);
+        return fdiv(r * r + r, r * r + T(1));
     } else if (LIM == PSST_LIMITER_MINMOD) {
         if (fabs(den) < eps) return T(1);
-        T r = num / den;
+        T r = fdiv(num, den);
         T res = r;
         if (r > T(1)) res = T(1);
         return res;
     } else {  // Van Leer
         if (fabs(den) < eps) return T(2);
-        T r = num / den;
-        return T(2) * r / (T(1) + r);
+        T r = fdiv(num, den);
+        return fdiv(T(2) * r, T(1) + r);
     }
 }
 
@@ -131,7 +192,7 @@ __device__ __forceinline__ T roe_avg(T lv, T rv, T ld, T rd) {
     T r = dsqrt(rd);
     T num = lv * l + rv * r;
     T den = (l + r);
-    return num / den;
+    return fdiv(num, den);
 }
 
 // ---------------------------------------------------------------------------
@@ -170,10 +231,7 @@ __global__ void __launch_bounds__(kBlock) k_grad_gg(Geom g, State<T> s) {
             for (int k = 0; k < DIM; ++k) acc[n][k] += sp * geom * nrm[k];
         }
     }
-#pragma unroll
-    for (int n = 0; n < NV; ++n)
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) s.grad[(size_t)c * GradLayout<DIM>::STRIDE + n * DIM + k] = acc[n][k];
+    store_grads<DIM, T>(s.grad, c, acc);
 }
 
 // psstLeastSquare::calculate (psstGradCalculation.cpp:268-368), per interior
@@ -206,10 +264,7 @@ __global__ void __launch_bounds__(kBlock) k_grad_lsq(Geom g, State<T> s) {
             for (int k = 0; k < DIM; ++k) acc[n][k] += w[k] * d;
         }
     }
-#pragma unroll
-    for (int n = 0; n < NV; ++n)
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) s.grad[(size_t)c * GradLayout<DIM>::STRIDE + n * DIM + k] = acc[n][k];
+    store_grads<DIM, T>(s.grad, c, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -248,8 +303,8 @@ __device__ __forceinline__ void reconstruct_dm(Side<DIM, T> &r0, Side<DIM, T> &r
         *p0 = v0 + T(0.5) * limiter<T, LIM>(nr, dnr) * dnr;
         *p1 = v1 + T(0.5) * limiter<T, LIM>(nl, dnl) * dnl;
     }
-    r0.rho = r0.p / (rm * r0.T_);
-    r1.rho = r1.p / (rm * r1.T_);
+    r0.rho = fdiv(r0.p, rm * r0.T_);
+    r1.rho = fdiv(r1.p, rm * r1.T_);
 }
 
 template <int DIM, typename T>
@@ -347,7 +402,7 @@ __device__ __forceinline__ void riemann_flux(const Side<DIM, T> (&r)[2], const T
                 side_state<DIM, T>(r[1], cv * r[1].T_ + T(0.5) * vsq2[1], wi1);
 #pragma unroll
                 for (int j = 0; j < WS; ++j)
-                    flow[j] = (s0 * fi1[j] - s1 * fi0[j] + s0 * s1 * (wi0[j] - wi1[j])) / (s0 - s1);
+                    flow[j] = fdiv(s0 * fi1[j] - s1 * fi0[j] + s0 * s1 * (wi0[j] - wi1[j]), s0 - s1);
             } else {
                 // s0 <= 0, or NaN speeds (the reference throws, quirk Q13; the
                 // NaN propagates into the state and psst_run reports it)
@@ -355,8 +410,8 @@ __device__ __forceinline__ void riemann_flux(const Side<DIM, T> (&r)[2], const T
             }
             return;
         }
-        T sh = (r[0].p - r[1].p + r[1].rho * vn[1] * (s1 - vn[1]) - r[0].rho * vn[0] * (s0 - vn[0])) /
-               (r[1].rho * (s1 - vn[1]) - r[0].rho * (s0 - vn[0]));
+        T sh = fdiv(r[0].p - r[1].p + r[1].rho * vn[1] * (s1 - vn[1]) - r[0].rho * vn[0] * (s0 - vn[0]),
+                    r[1].rho * (s1 - vn[1]) - r[0].rho * (s0 - vn[0]));
         if ((s1 <= T(0) && sh >= T(0)) || (sh <= T(0) && s0 >= T(0))) {
             const int K = (s1 <= T(0) && sh >= T(0)) ? 1 : 0;
             const Side<DIM, T> &rk = r[K];
@@ -368,16 +423,16 @@ __device__ __forceinline__ void riemann_flux(const Side<DIM, T> (&r)[2], const T
             const T tt = sK - sh;
             const T PK = rk.p + rk.rho * (sK - vK) * (sh - vK);
 #pragma unroll
-            for (int j = 0; j < DIM; ++j) flow[j] = (sh * (sK * wiK[j] - fiK[j]) + sK * PK * nrm[j]) / tt;
-            flow[DIM] = (sh * (sK * wiK[DIM] - fiK[DIM])) / tt;
-            flow[DIM + 1] = (sh * (sK * wiK[DIM + 1] - fiK[DIM + 1]) + sK * PK * sh) / tt;
+            for (int j = 0; j < DIM; ++j) flow[j] = fdiv(sh * (sK * wiK[j] - fiK[j]) + sK * PK * nrm[j], tt);
+            flow[DIM] = fdiv(sh * (sK * wiK[DIM] - fiK[DIM]), tt);
+            flow[DIM + 1] = fdiv(sh * (sK * wiK[DIM + 1] - fiK[DIM + 1]) + sK * PK * sh, tt);
         } else {
             side_flux<DIM, T>(r[0], vn[0], H[0], nrm, flow);
         }
     } else {
         T M[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) M[j] = vn[j] / a[j];
+        for (int j = 0; j < 2; ++j) M[j] = fdiv(vn[j], a[j]);
         T Ms1, ps1, Ms0, ps0;
         if (M[1] >= T(1)) {
             Ms1 = M[1];
@@ -481,7 +536,7 @@ __device__ __forceinline__ T dt_local1(const Geom &g, int c, const T *uc, T Tc, 
         nn = (nn + nv) * T(0.5);
         den += (fabs(nn) + cn) * T(g.fa[f]);
     }
-    return T(ph.cfl) * T(g.vol[c]) / den;
+    return fdiv(T(ph.cfl) * T(g.vol[c]), den);
 }
 
 // psstLocalAccTime2::calculate_at (psstLocalTimeCalculation.cpp:175-213);
@@ -492,7 +547,7 @@ __device__ __forceinline__ T dt_local2(const Geom &g, int c, const T *uc, T Tc, 
     T den = T(0);
 #pragma unroll
     for (int j = 0; j < DIM; ++j) den += (fabs(uc[j]) + cc) * T(g.S[(size_t)j * g.N + c]) * T(0.5);
-    return T(ph.cfl) * T(g.vol[c]) / den;
+    return fdiv(T(ph.cfl) * T(g.vol[c]), den);
 }
 
 template <typename T>
@@ -562,7 +617,7 @@ __device__ __forceinline__ void update_cell(const Geom &g, const State<T> &s, co
 #pragma unroll
     for (int j = 0; j < WS; ++j) {
         T w0 = s.W0[(size_t)j * g.N + c];
-        w[j] = w0 - dt * alpha * R[j] / V;
+        w[j] = w0 - fdiv(dt * alpha * R[j], V);
         unsigned long long ab = abs_bits(R[j]);
         rmax[j] = rmax[j] > ab ? rmax[j] : ab;
     }
@@ -574,11 +629,11 @@ __device__ __forceinline__ void update_cell(const Geom &g, const State<T> &s, co
     T vsq = T(0);
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
-        T u = w[k] / rho;
+        T u = fdiv(w[k], rho);
         s.qn[(size_t)k * g.NT + c] = u;
         vsq += u * u;
     }
-    T Tn = ((w[DIM + 1] / rho) - (vsq * T(0.5))) / T(ph.cv);
+    T Tn = fdiv(fdiv(w[DIM + 1], rho) - (vsq * T(0.5)), T(ph.cv));
     s.qn[(size_t)DIM * g.NT + c] = rho * T(ph.rm) * Tn;
     s.qn[(size_t)(DIM + 1) * g.NT + c] = Tn;
     s.qn[(size_t)(DIM + 2) * g.NT + c] = rho;
@@ -733,7 +788,7 @@ __device__ __forceinline__ void bc_apply(const Geom &g, T *__restrict__ qn, cons
     for (int k = 0; k < DIM; ++k) qn[k * NT + gc] = u[k];
     qn[DIM * NT + gc] = p;
     qn[(DIM + 1) * NT + gc] = Tt;
-    qn[(DIM + 2) * NT + gc] = p / (Tt * T(ph.rm));
+    qn[(DIM + 2) * NT + gc] = fdiv(p, Tt * T(ph.rm));
 }
 
 template <int DIM, typename T>
@@ -824,7 +879,7 @@ struct TileDev {
     const double *__restrict__ beta;     // [TF]
     const int *__restrict__ ghost_off;   // [T+1]
     const int *__restrict__ ghost;       // ghost device ids
-    const short *__restrict__ visit;     // [Wv][N] local face, face-loop order
+    const short *__restrict__ visit;     // [Wv][N] local face | 0x4000 if neighbour 0, face-loop order
     const short *__restrict__ list;      // [Wl][N] local face, face-list order
     const double *__restrict__ lsqc;     // [DIM][Wl][N] least-squares weights
     int TF, Wv, Wl, MS, MF;              // total tile faces, ELL widths, LDS slot/face capacity
@@ -845,40 +900,20 @@ __device__ __forceinline__ int end1(unsigned e) { return (int)((e >> 15) & 0x7ff
 __device__ __forceinline__ int endkind(unsigned e) { return (int)(e >> 30); }
 
 // Stage the tile's slots: q (DIM+3 rows) and the slot -> cell map.
-template <int DIM, typename T>
+// With second order an interior cell's density is never read (face states
+// take rho from p and T), so only ghost slots load it.
+template <int DIM, typename T, bool ALL_RHO>
 __device__ __forceinline__ void stage_slots(const Geom &g, const TileDev &td, const State<T> &s, int cb, int nc, int hb,
                                             int ns, T *sq, int *scell) {
-    constexpr int NQ = DIM + 3;
     for (int i = threadIdx.x; i < ns; i += blockDim.x) {
         const int cell = i < nc ? cb + i : td.halo[hb + i - nc];
         scell[i] = cell;
 #pragma unroll
-        for (int v = 0; v < NQ; ++v) sq[v * td.MS + i] = s.q[(size_t)v * g.NT + cell];
+        for (int v = 0; v < DIM + 2; ++v) sq[v * td.MS + i] = s.q[(size_t)v * g.NT + cell];
+        if (ALL_RHO || cell >= g.N) sq[(DIM + 2) * td.MS + i] = s.q[(size_t)(DIM + 2) * g.NT + cell];
     }
 }
 
-// Gradients of one cell (cell-major layout) into registers, wide loads.
-template <int DIM, typename T>
-__device__ __forceinline__ void load_grads(const T *__restrict__ gr, int cell, T (&g)[GradLayout<DIM>::STRIDE]) {
-    constexpr int S = GradLayout<DIM>::STRIDE;
-    if (sizeof(T) == 8) {
-        const double2 *p = reinterpret_cast<const double2 *>(gr + (size_t)cell * S);
-#pragma unroll
-        for (int i = 0; i < S / 2; ++i) {
-            double2 v = p[i];
-            g[2 * i] = (T)v.x;
-            g[2 * i + 1] = (T)v.y;
-        }
-    } else {
-        const float2 *p = reinterpret_cast<const float2 *>(gr + (size_t)cell * S);
-#pragma unroll
-        for (int i = 0; i < S / 2; ++i) {
-            float2 v = p[i];
-            g[2 * i] = (T)v.x;
-            g[2 * i + 1] = (T)v.y;
-        }
-    }
-}
 
 // One RK stage, fused: face fluxes -> per-cell ordered gather -> time step ->
 // update -> primitives -> ghost states of the tile's boundary faces.
@@ -895,7 +930,21 @@ __global__ void __launch_bounds__(NTHR) k_stage(Geom g, TileDev td, State<T> s, 
     const int cb = td.cell_begin[t], nc = td.cell_begin[t + 1] - cb;
     const int hb = td.halo_off[t], ns = nc + (td.halo_off[t + 1] - hb);
     const int fb = td.face_off[t], nf = td.face_off[t + 1] - fb;
-    stage_slots<DIM, T>(g, td, s, cb, nc, hb, ns, sq, scell);
+    // prefetch the update inputs of this thread's first cell; they arrive
+    // while the slots load and the faces are computed
+    const int c_pre = cb + (int)threadIdx.x;
+    const bool has_pre = (int)threadIdx.x < nc;
+    T w0_pre[WS];
+    double V_pre = 0.0, S_pre[DIM];
+    short ve_pre[4];
+#pragma unroll
+    for (int j = 0; j < WS; ++j) w0_pre[j] = has_pre ? s.W0[(size_t)j * g.N + c_pre] : T(0);
+    V_pre = has_pre ? g.vol[c_pre] : 1.0;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) S_pre[k] = (has_pre && LC == PSST_DT_LOCAL2) ? g.S[(size_t)k * g.N + c_pre] : 0.0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) ve_pre[w] = (has_pre && w < td.Wv) ? td.visit[(size_t)w * g.N + c_pre] : (short)-1;
+    stage_slots<DIM, T, ORDER == 1>(g, td, s, cb, nc, hb, ns, sq, scell);
     __syncthreads();
     // ---- faces of the tile ----
     for (int j = threadIdx.x; j < nf; j += blockDim.x) {
@@ -951,13 +1000,15 @@ __global__ void __launch_bounds__(NTHR) k_stage(Geom g, TileDev td, State<T> s, 
     for (int j = 0; j < WS; ++j) rmax[j] = 0;
     for (int i = threadIdx.x; i < nc; i += blockDim.x) {
         const int c = cb + i;
+        const bool pre = (i == (int)threadIdx.x);
         T R[WS];
 #pragma unroll
         for (int j = 0; j < WS; ++j) R[j] = T(0);
         for (int w = 0; w < td.Wv; ++w) {
-            const int lf = td.visit[(size_t)w * g.N + c];
-            if (lf < 0) break;
-            const bool first = end0(td.ends[fb + lf]) == i;
+            const int e = (pre && w < 4) ? ve_pre[w] : td.visit[(size_t)w * g.N + c];
+            if (e < 0) break;
+            const int lf = e & 0x3fff;
+            const bool first = (e & 0x4000) != 0;
 #pragma unroll
             for (int j = 0; j < WS; ++j) {
                 const T v = sfa[j * td.MF + lf];
@@ -972,7 +1023,16 @@ __global__ void __launch_bounds__(NTHR) k_stage(Geom g, TileDev td, State<T> s, 
         if (LC == PSST_DT_UNIFORM) {
             dt = T(ph.dt_uniform);
         } else if (LC == PSST_DT_LOCAL2) {
-            dt = dt_local2<DIM, T>(g, c, uc, Tc, ph);
+            if (pre) {
+                // dt_local2 with the prefetched projected areas
+                T cc = dsqrt(T(ph.gamma) * T(ph.rm) * Tc);
+                T den = T(0);
+#pragma unroll
+                for (int j = 0; j < DIM; ++j) den += (fabs(uc[j]) + cc) * T(S_pre[j]) * T(0.5);
+                dt = fdiv(T(ph.cfl) * T(V_pre), den);
+            } else {
+                dt = dt_local2<DIM, T>(g, c, uc, Tc, ph);
+            }
         } else {
             // time step mode 1 with every neighbour at the stage-start state
             // (Jacobi; psstLocalTimeCalculation.cpp:90-131 otherwise)
@@ -995,14 +1055,14 @@ __global__ void __launch_bounds__(NTHR) k_stage(Geom g, TileDev td, State<T> s, 
                 nn = (nn + nv) * T(0.5);
                 den += (fabs(nn) + cn) * T(td.fa[fb + lf]);
             }
-            dt = T(ph.cfl) * T(g.vol[c]) / den;
+            dt = fdiv(T(ph.cfl) * T(g.vol[c]), den);
         }
-        const T V = T(g.vol[c]);
+        const T V = T(pre ? V_pre : g.vol[c]);
         T w[WS];
 #pragma unroll
         for (int j = 0; j < WS; ++j) {
-            T w0 = s.W0[(size_t)j * g.N + c];
-            w[j] = w0 - dt * alpha * R[j] / V;
+            T w0 = pre ? w0_pre[j] : s.W0[(size_t)j * g.N + c];
+            w[j] = w0 - fdiv(dt * alpha * R[j], V);
             unsigned long long ab = abs_bits(R[j]);
             rmax[j] = rmax[j] > ab ? rmax[j] : ab;
         }
@@ -1013,11 +1073,11 @@ __global__ void __launch_bounds__(NTHR) k_stage(Geom g, TileDev td, State<T> s, 
         T vsq = T(0);
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
-            T u = w[k] / rho;
+            T u = fdiv(w[k], rho);
             s.qn[(size_t)k * g.NT + c] = u;
             vsq += u * u;
         }
-        T Tn = ((w[DIM + 1] / rho) - (vsq * T(0.5))) / T(ph.cv);
+        T Tn = fdiv(fdiv(w[DIM + 1], rho) - (vsq * T(0.5)), T(ph.cv));
         s.qn[(size_t)DIM * g.NT + c] = rho * T(ph.rm) * Tn;
         s.qn[(size_t)(DIM + 1) * g.NT + c] = Tn;
         s.qn[(size_t)(DIM + 2) * g.NT + c] = rho;
@@ -1057,8 +1117,9 @@ __global__ void __launch_bounds__(NTHR) k_grad_gg_tile(Geom g, TileDev td, State
             for (int k = 0; k < DIM; ++k) acc[n][k] = T(0);
         const double Vc = g.vol[c];
         for (int w = 0; w < td.Wv; ++w) {
-            const int lf = td.visit[(size_t)w * g.N + c];
-            if (lf < 0) break;
+            const int ve = td.visit[(size_t)w * g.N + c];
+            if (ve < 0) break;
+            const int lf = ve & 0x3fff;
             const unsigned e = td.ends[fb + lf];
             const int l0 = end0(e), l1 = end1(e);
             const T geom = T(td.fa[fb + lf] / Vc);
@@ -1076,10 +1137,7 @@ __global__ void __launch_bounds__(NTHR) k_grad_gg_tile(Geom g, TileDev td, State
                 for (int k = 0; k < DIM; ++k) acc[n][k] += sp * geom * nrm[k];
             }
         }
-#pragma unroll
-        for (int n = 0; n < NV; ++n)
-#pragma unroll
-            for (int k = 0; k < DIM; ++k) s.grad[(size_t)c * GradLayout<DIM>::STRIDE + n * DIM + k] = acc[n][k];
+        store_grads<DIM, T>(s.grad, c, acc);
     }
 }
 
@@ -1123,10 +1181,7 @@ __global__ void __launch_bounds__(NTHR) k_grad_lsq_tile(Geom g, TileDev td, Stat
                 for (int k = 0; k < DIM; ++k) acc[n][k] += wk[k] * d;
             }
         }
-#pragma unroll
-        for (int n = 0; n < NV; ++n)
-#pragma unroll
-            for (int k = 0; k < DIM; ++k) s.grad[(size_t)c * GradLayout<DIM>::STRIDE + n * DIM + k] = acc[n][k];
+        store_grads<DIM, T>(s.grad, c, acc);
     }
 }
 
@@ -1767,11 +1822,14 @@ psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, pss
         // ---- tile plan of the fused kernels ----
         const bool ordered_dt1 = (s->time_mode == PSST_DT_LOCAL1 && s->dt_ordering == 0);
         if (s->fused && !ordered_dt1) {
-            const int tile = s->tile_cells > 0 ? s->tile_cells : 256;
+            // defaults measured on the 3840x1280 wedge (HLLC, second order):
+            // FP64 512 cells / 512 threads, FP32 256 / 256
+            const int def_tile = s->precision == 64 ? 512 : 256;
+            const int tile = s->tile_cells > 0 ? s->tile_cells : def_tile;
             if (tile < 32 || tile > 4096) throw std::invalid_argument("tile_cells must be 32..4096");
             if (s->tile_threads != 0 && s->tile_threads != 256 && s->tile_threads != 512)
                 throw std::invalid_argument("tile_threads must be 256 or 512");
-            c->tile_threads = s->tile_threads ? s->tile_threads : 256;
+            c->tile_threads = s->tile_threads ? s->tile_threads : (tile >= 512 ? 512 : 256);
             psst::TilePlan tp = psst::plan_tiles(N, G, F, tile, f0d.data(), f1d.data(), voff.data(), vface.data(),
                                                  loff.data(), lface.data());
             const int NQ = dim + 3, NV = dim + 2;

@@ -108,7 +108,7 @@ TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t
         }
         const int32_t ns = nc + (int32_t)halo_cells.size();
         const int32_t nf = (int32_t)tile_faces.size();
-        if (ns >= 32768 || nf >= 32768) throw std::runtime_error("tile too large for 15-bit local indices");
+        if (ns >= 32768 || nf >= 16384) throw std::runtime_error("tile too large for its local indices");
         p.max_slots = std::max(p.max_slots, ns);
         p.max_faces = std::max(p.max_faces, nf);
         for (int32_t f : tile_faces) {
@@ -125,8 +125,10 @@ TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t
         p.max_ghosts = std::max(p.max_ghosts, ng);
         for (int32_t c = b; c < e; ++c) {
             int32_t w = 0;
-            for (int32_t k = voff[c]; k < voff[c + 1]; ++k, ++w)
-                p.visit_ell[(size_t)w * N + c] = (int16_t)lface_of[(size_t)vface[k]];
+            for (int32_t k = voff[c]; k < voff[c + 1]; ++k, ++w) {
+                const int32_t f = vface[k];
+                p.visit_ell[(size_t)w * N + c] = (int16_t)(lface_of[(size_t)f] | (f0[f] == c ? 0x4000 : 0));
+            }
             w = 0;
             for (int32_t k = loff[c]; k < loff[c + 1]; ++k, ++w) {
                 int32_t lf = lface_of[(size_t)lface[k]];

@@ -34,7 +34,8 @@ struct TilePlan {
                                         // kind: 0 interior, 1 ghost is neighbour 0, 2 ghost is neighbour 1
     std::vector<int32_t> ghost_off;     // [T+1]
     std::vector<int32_t> ghost;         // ghost device ids updated by the tile
-    std::vector<int16_t> visit_ell;     // [W_visit][N] local face ids, face-loop order, -1 padded
+    std::vector<int16_t> visit_ell;     // [W_visit][N] local face id | 0x4000 if the cell is the face's
+                                        // neighbour 0; face-loop order, -1 padded
     std::vector<int16_t> list_ell;      // [W_list][N] local face ids, face-list order, -1 padded
 };
 
