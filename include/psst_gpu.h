@@ -180,6 +180,62 @@ void psst_ctx_destroy(psst_ctx *ctx);
 /* Last error of a context (or of the last failed create when ctx is NULL). */
 const char *psst_last_error(const psst_ctx *ctx);
 
+/* Switches the state between FP64 and FP32 in place (geometry and residual
+ * norms stay FP64).  An FP32 state cannot drive the residual much below
+ * about 1e-4 of its first value (the rounding of the face-flux sums), so a
+ * solve to a tight tolerance can run FP32 first and finish in FP64.  No
+ * reference counterpart (the reference is FP64 only). */
+psst_status psst_set_precision(psst_ctx *ctx, int32_t precision);
+
+/* Autotuner of the fused kernels' tile shape: runs `trial_iters` iterations
+ * of the given mesh, settings and state for each candidate (cells per tile x
+ * threads per workgroup) and returns the fastest in best[0..1].  cand
+ * ([max_cand][2]) and ms_per_iter ([max_cand]) receive every candidate and
+ * its time (-1: did not run).  No reference counterpart. */
+psst_status psst_autotune(const psst_mesh_desc *mesh, const psst_settings *settings, const double *prim,
+                          int32_t trial_iters, int32_t max_cand, int32_t *cand, double *ms_per_iter,
+                          int32_t *n_cand, int32_t *best);
+
+/* ---------------------------------------------------------------------------
+ * Batches: many independent instances (meshes, boundary values) advanced
+ * together, one launch per kernel for all of them.  The reference solves one
+ * case per process (main.cpp:9-63); a batch is the same explicit_solve loop
+ * (psstFiniteVolumeGrid.cpp:305-397) over every instance, each instance
+ * bitwise identical to its own psst_ctx run in FP64.  The instances share
+ * the scheme (every psst_settings field but the boundary table); they are
+ * stored as one union mesh whose tiles never straddle two instances.  After
+ * every iteration the device computes each instance's residual norms and
+ * freezes the instances that reached the tolerance; the remaining tiles are
+ * served to a persistent grid from per-XCD work queues, and each iteration
+ * is replayed from a captured hipGraph.
+ * ------------------------------------------------------------------------- */
+typedef struct psst_batch psst_batch;
+
+psst_status psst_batch_create(int32_t n, const psst_mesh_desc *meshes, const psst_settings *settings,
+                              psst_batch **out);
+/* Per-instance state, in the instance's own canonical numbering
+ * ([(dim+3)][N_i+G_i], as psst_set_state / psst_get_state). */
+psst_status psst_batch_set_state(psst_batch *b, int32_t inst, const double *prim);
+psst_status psst_batch_get_state(psst_batch *b, int32_t inst, double *prim);
+psst_status psst_batch_set_residual_reference(psst_batch *b, int32_t inst, const double *res0);
+/* Runs every instance (those with a finite state) for up to max_iter
+ * iterations; with tol > 0 each stops after its first iteration whose
+ * max_j res_j / res0_j <= tol.  Per instance (each may be NULL): iters =
+ * iterations run, status = PSST_OK / PSST_NO_CONVERGENCE / PSST_NAN_DETECTED,
+ * drop = last max_j res_j / res0_j; res_hist = [max_iter][n][dim+2] residual
+ * rows (zero once an instance stopped).  Returns the worst status. */
+psst_status psst_batch_run(psst_batch *b, int32_t max_iter, double tol, int32_t *iters, int32_t *status,
+                           double *drop, double *res_hist);
+psst_status psst_batch_set_precision(psst_batch *b, int32_t precision);
+/* 1: replay iterations from a captured hipGraph (default); 0: launch each kernel. */
+psst_status psst_batch_set_graph(psst_batch *b, int32_t enable);
+/* info[0..7] = instances, union N, G, F, tiles, cells per tile, threads per tile, iterations run */
+psst_status psst_batch_info(const psst_batch *b, int64_t *info);
+/* The batch's device context (for psst_kernel_timing / psst_kernel_stats). */
+psst_ctx *psst_batch_context(psst_batch *b);
+void psst_batch_destroy(psst_batch *b);
+const char *psst_batch_last_error(const psst_batch *b);
+
 /* ---------------------------------------------------------------------------
  * Host-side helpers of the same library (the build's own host C++): Fluent
  * reader, settings files, wedge generator.  They replace main.cpp's setup

@@ -19,7 +19,8 @@ import numpy as np
 
 from ._lib import BC, MeshDesc, PsstError, Settings, WedgeParams, lib
 
-__all__ = ["Case", "Solver", "PsstError", "WedgeParams", "wedge_params", "write_wedge_msh", "write_case_files"]
+__all__ = ["Case", "Solver", "Batch", "PsstError", "WedgeParams", "wedge_params", "write_wedge_msh",
+           "write_case_files", "autotune"]
 
 
 def _dp(a):
@@ -189,7 +190,11 @@ class Solver:
         self.case = case
         s = settings if settings is not None else case.settings()
         for k, v in overrides.items():
-            setattr(s, k, v)
+            if k == "rk_coeff":
+                for j, a in enumerate(v):
+                    s.rk_coeff[j] = a
+            else:
+                setattr(s, k, v)
         self.settings = s
         self.dim = case.dim
         self.n_cells, self.n_ghosts = case.n_cells, case.n_ghosts
@@ -268,7 +273,150 @@ class Solver:
         _check(lib().psst_kernel_stats(self._h, name.encode(), C.byref(cnt), C.byref(ms)), self._h)
         return int(cnt.value), float(ms.value)
 
+    def set_precision(self, precision):
+        """Switches the device state between FP64 and FP32 in place."""
+        _check(lib().psst_set_precision(self._h, int(precision)), self._h)
+
     def kernel_names(self):
         buf = C.create_string_buffer(4096)
         _check(lib().psst_kernel_names(self._h, buf, 4096), self._h)
         return buf.value.decode().split(";")
+
+
+class Batch:
+    """Many independent cases advanced together on the GPU (``psst_batch_*``):
+    one union mesh, one launch per kernel per stage for all instances, each
+    instance frozen on the device once it reaches the tolerance.  Instances
+    share the scheme (every ``psst_settings`` field except the boundary
+    table) and may differ in mesh and boundary values.  In FP64 every
+    instance is bitwise identical to its own ``Solver`` run."""
+
+    def __init__(self, cases, settings=None, **overrides):
+        self.cases = list(cases)
+        n = len(self.cases)
+        if n == 0:
+            raise ValueError("empty batch")
+        self.dim = self.cases[0].dim
+        self._meshes = (MeshDesc * n)()
+        self._settings = (Settings * n)()
+        for i, c in enumerate(self.cases):
+            self._meshes[i] = c.mesh()
+            s = c.settings()
+            if settings is not None:
+                for name, _ in Settings._fields_:
+                    if name not in ("bc", "n_bc"):
+                        setattr(s, name, getattr(settings, name))
+            for k, v in overrides.items():
+                if k == "rk_coeff":
+                    for j, a in enumerate(v):
+                        s.rk_coeff[j] = a
+                else:
+                    setattr(s, k, v)
+            self._settings[i] = s
+        h = C.c_void_p()
+        st = lib().psst_batch_create(n, self._meshes, self._settings, C.byref(h))
+        if st != 0:
+            raise PsstError(st, lib().psst_batch_last_error(None).decode())
+        self._h = h
+        self.n = n
+        for i, c in enumerate(self.cases):
+            self.set_state(i, c.initial_state())
+
+    def _check(self, st):
+        if st != 0:
+            raise PsstError(st, lib().psst_batch_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().psst_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_state(self, i, prim):
+        c = self.cases[i]
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        assert prim.shape == (self.dim + 3, c.n_cells + c.n_ghosts)
+        self._check(lib().psst_batch_set_state(self._h, int(i), _dp(prim)))
+
+    def state(self, i):
+        c = self.cases[i]
+        prim = np.zeros((self.dim + 3, c.n_cells + c.n_ghosts))
+        self._check(lib().psst_batch_get_state(self._h, int(i), _dp(prim)))
+        return prim
+
+    def set_residual_reference(self, i, res0):
+        r = np.ascontiguousarray(res0, dtype=np.float64)
+        self._check(lib().psst_batch_set_residual_reference(self._h, int(i), _dp(r)))
+
+    def set_precision(self, precision):
+        self._check(lib().psst_batch_set_precision(self._h, int(precision)))
+
+    def set_graph(self, enable):
+        self._check(lib().psst_batch_set_graph(self._h, int(bool(enable))))
+
+    def info(self):
+        a = (C.c_int64 * 8)()
+        self._check(lib().psst_batch_info(self._h, a))
+        keys = ("instances", "n_cells", "n_ghosts", "n_faces", "tiles", "tile_cells", "tile_threads", "iterations")
+        return dict(zip(keys, (int(x) for x in a)))
+
+    def run(self, max_iter, tol=0.0, history=False, allow_no_convergence=False):
+        """Runs every instance for up to max_iter iterations (each stops at its
+        first iteration with max_j res_j/res0_j <= tol when tol > 0).  Returns a
+        dict of per-instance ``iters``, ``status``, ``drop`` and, with
+        history=True, ``hist`` [max_iter, n, dim+2]."""
+        n = self.n
+        iters = np.zeros(n, dtype=np.int32)
+        status = np.zeros(n, dtype=np.int32)
+        drop = np.zeros(n)
+        hist = np.zeros((max(int(max_iter), 1), n, self.dim + 2)) if history else None
+        st = lib().psst_batch_run(self._h, int(max_iter), float(tol), iters.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  status.ctypes.data_as(C.POINTER(C.c_int32)), _dp(drop),
+                                  _dp(hist) if history else None)
+        out = {"iters": iters, "status": status, "drop": drop}
+        if history:
+            out["hist"] = hist[:max_iter]
+        if st == 4 and allow_no_convergence:
+            return out
+        self._check(st)
+        return out
+
+    def kernel_timing(self, enable=True):
+        ctx = lib().psst_batch_context(self._h)
+        _check(lib().psst_kernel_timing(ctx, int(bool(enable))), ctx)
+
+    def kernel_stats(self, name):
+        ctx = lib().psst_batch_context(self._h)
+        cnt = C.c_int64(0)
+        ms = C.c_double(0)
+        _check(lib().psst_kernel_stats(ctx, name.encode(), C.byref(cnt), C.byref(ms)), ctx)
+        return int(cnt.value), float(ms.value)
+
+
+def autotune(case, settings=None, prim=None, trial_iters=10, **overrides):
+    """Times the fused kernels' tile shapes on a case (``psst_autotune``) and
+    returns (best (cells, threads), [((cells, threads), ms per iteration), ...])."""
+    s = settings if settings is not None else case.settings()
+    for k, v in overrides.items():
+        setattr(s, k, v)
+    m = case.mesh()
+    prim = np.ascontiguousarray(case.initial_state() if prim is None else prim, dtype=np.float64)
+    cand = (C.c_int32 * 64)()
+    ms = (C.c_double * 32)()
+    ncand = C.c_int32(0)
+    best = (C.c_int32 * 2)()
+    _check(lib().psst_autotune(C.byref(m), C.byref(s), _dp(prim), int(trial_iters), 32, cand, ms, C.byref(ncand),
+                               best))
+    table = [((cand[2 * k], cand[2 * k + 1]), ms[k]) for k in range(ncand.value)]
+    return (best[0], best[1]), table

@@ -89,6 +89,22 @@ _SIGS = {
     "psst_case_solve": (C.c_int, [C.c_void_p]),
     "psst_case_close": (None, [C.c_void_p]),
     "psst_case_last_error": (C.c_char_p, []),
+    "psst_set_precision": (C.c_int, [C.c_void_p, C.c_int32]),
+    "psst_autotune": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(Settings), C.POINTER(C.c_double), C.c_int32, C.c_int32,
+                                C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_int32),
+                                C.POINTER(C.c_int32)]),
+    "psst_batch_create": (C.c_int, [C.c_int32, C.POINTER(MeshDesc), C.POINTER(Settings), C.POINTER(C.c_void_p)]),
+    "psst_batch_set_state": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "psst_batch_get_state": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "psst_batch_set_residual_reference": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "psst_batch_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "psst_batch_set_precision": (C.c_int, [C.c_void_p, C.c_int32]),
+    "psst_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
+    "psst_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "psst_batch_context": (C.c_void_p, [C.c_void_p]),
+    "psst_batch_destroy": (None, [C.c_void_p]),
+    "psst_batch_last_error": (C.c_char_p, [C.c_void_p]),
 }
 
 

@@ -895,6 +895,60 @@ __device__ __forceinline__ int tile_of_block(int n_tiles) {
     return (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
 }
 
+// Work list of a batched launch.  A batch holds many independent instances
+// in one union mesh; its tiles never straddle two instances.  The tiles of
+// the instances still iterating form the list [0, *count); a persistent
+// grid pops them from 8 queues, one per XCD, each a contiguous eighth of the
+// list (neighbouring tiles share halo cells, so they are best served by the
+// same L2), and steals from the other queues once its own is empty.
+// list == nullptr: one tile per workgroup, mapped by tile_of_block.
+struct BatchDev {
+    const int *__restrict__ list;       // active tiles
+    const int *__restrict__ count;      // number of active tiles
+    int *ctr;                           // [8] queue heads of this launch (zeroed before it)
+    const int *__restrict__ tile_inst;  // [n_tiles] instance of each tile
+    unsigned long long *res;            // [n_inst][dim+2][kResSlots] residual slots of the iteration
+};
+
+__device__ __forceinline__ int pop_item(const BatchDev &bd) {
+    __shared__ int s_item;
+    __syncthreads();  // every thread has read the previous item
+    if (threadIdx.x == 0) {
+        const int cnt = *bd.count;
+        const int per = (cnt + 7) >> 3;
+        const int x0 = (int)(blockIdx.x & 7u);
+        int item = -1;
+        for (int q = 0; q < 8 && item < 0; ++q) {
+            const int x = (x0 + q) & 7;
+            const int b = x * per, n = min(cnt, b + per) - b;
+            if (n <= 0 || __hip_atomic_load(&bd.ctr[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n) continue;
+            const int k = atomicAdd(&bd.ctr[x], 1);
+            if (k < n) item = b + k;
+        }
+        s_item = item;
+    }
+    __syncthreads();
+    return s_item;
+}
+
+// Drives a per-tile body over this workgroup's tiles: the one tile of an
+// unbatched launch, or the popped items of a batched one.
+template <int WS, typename Body>
+__device__ __forceinline__ void for_each_tile(const TileDev &td, const BatchDev &bd, unsigned long long *res_row,
+                                              Body body) {
+    if (!bd.list) {
+        const int t = tile_of_block(td.n_tiles);
+        if (t < td.n_tiles) body(t, res_row);
+        return;
+    }
+    for (;;) {
+        const int item = pop_item(bd);
+        if (item < 0) return;
+        const int t = bd.list[item];
+        body(t, bd.res + (size_t)bd.tile_inst[t] * WS * kResSlots);
+    }
+}
+
 __device__ __forceinline__ int end0(unsigned e) { return (int)(e & 0x7fffu); }
 __device__ __forceinline__ int end1(unsigned e) { return (int)((e >> 15) & 0x7fffu); }
 __device__ __forceinline__ int endkind(unsigned e) { return (int)(e >> 30); }
@@ -917,16 +971,14 @@ __device__ __forceinline__ void stage_slots(const Geom &g, const TileDev &td, co
 
 // One RK stage, fused: face fluxes -> per-cell ordered gather -> time step ->
 // update -> primitives -> ghost states of the tile's boundary faces.
-template <int DIM, typename T, int FLUX, int ORDER, int LIM, int LC, int NTHR>
-__global__ void __launch_bounds__(NTHR) k_stage(Geom g, TileDev td, State<T> s, Phys ph, T alpha, int last_stage,
-                                                  unsigned long long *res_row, const BCDev *__restrict__ bcs) {
-    extern __shared__ __align__(16) unsigned char smem[];
+template <int DIM, typename T, int FLUX, int ORDER, int LIM, int LC>
+__device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, const State<T> &s, const Phys &ph, T alpha,
+                                           int last_stage, unsigned long long *res_row, const BCDev *__restrict__ bcs,
+                                           int t, unsigned char *smem) {
     constexpr int NQ = DIM + 3, NV = DIM + 2, WS = DIM + 2;
     T *sq = reinterpret_cast<T *>(smem);
     T *sfa = sq + NQ * td.MS;
     int *scell = reinterpret_cast<int *>(sfa + NV * td.MF);
-    const int t = tile_of_block(td.n_tiles);
-    if (t >= td.n_tiles) return;
     const int cb = td.cell_begin[t], nc = td.cell_begin[t + 1] - cb;
     const int hb = td.halo_off[t], ns = nc + (td.halo_off[t + 1] - hb);
     const int fb = td.face_off[t], nf = td.face_off[t + 1] - fb;
@@ -1089,16 +1141,24 @@ __global__ void __launch_bounds__(NTHR) k_stage(Geom g, TileDev td, State<T> s, 
     residual_max<WS>(rmax, res_row);
 }
 
+template <int DIM, typename T, int FLUX, int ORDER, int LIM, int LC, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_stage(Geom g, TileDev td, State<T> s, Phys ph, T alpha, int last_stage,
+                                                  unsigned long long *res_row, const BCDev *__restrict__ bcs,
+                                                  BatchDev bd) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    for_each_tile<DIM + 2>(td, bd, res_row, [&](int t, unsigned long long *row) {
+        stage_tile<DIM, T, FLUX, ORDER, LIM, LC>(g, td, s, ph, alpha, last_stage, row, bcs, t, smem);
+    });
+}
+
 // Green-Gauss gradients on a tile (psstGreenGauss::calculate,
 // psstGradCalculation.cpp:80-177): q of own and halo cells in LDS, each
 // cell's faces in face-loop order.
-template <int DIM, typename T, int NTHR>
-__global__ void __launch_bounds__(NTHR) k_grad_gg_tile(Geom g, TileDev td, State<T> s) {
-    extern __shared__ __align__(16) unsigned char smem[];
+template <int DIM, typename T>
+__device__ __forceinline__ void grad_gg_tile_body(const Geom &g, const TileDev &td, const State<T> &s, int t,
+                                            unsigned char *smem) {
     constexpr int NV = DIM + 2;
     T *sq = reinterpret_cast<T *>(smem);
-    const int t = tile_of_block(td.n_tiles);
-    if (t >= td.n_tiles) return;
     const int cb = td.cell_begin[t], nc = td.cell_begin[t + 1] - cb;
     const int hb = td.halo_off[t], ns = nc + (td.halo_off[t + 1] - hb);
     const int fb = td.face_off[t];
@@ -1141,15 +1201,21 @@ __global__ void __launch_bounds__(NTHR) k_grad_gg_tile(Geom g, TileDev td, State
     }
 }
 
+template <int DIM, typename T, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_grad_gg_tile(Geom g, TileDev td, State<T> s, BatchDev bd) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    for_each_tile<DIM + 2>(td, bd, nullptr, [&](int t, unsigned long long *) {
+        grad_gg_tile_body<DIM, T>(g, td, s, t, smem);
+    });
+}
+
 // Least-squares gradients on a tile (psstLeastSquare::calculate,
 // psstGradCalculation.cpp:268-368), face-list order.
-template <int DIM, typename T, int NTHR>
-__global__ void __launch_bounds__(NTHR) k_grad_lsq_tile(Geom g, TileDev td, State<T> s) {
-    extern __shared__ __align__(16) unsigned char smem[];
+template <int DIM, typename T>
+__device__ __forceinline__ void grad_lsq_tile_body(const Geom &g, const TileDev &td, const State<T> &s, int t,
+                                            unsigned char *smem) {
     constexpr int NV = DIM + 2;
     T *sq = reinterpret_cast<T *>(smem);
-    const int t = tile_of_block(td.n_tiles);
-    if (t >= td.n_tiles) return;
     const int cb = td.cell_begin[t], nc = td.cell_begin[t + 1] - cb;
     const int hb = td.halo_off[t], ns = nc + (td.halo_off[t + 1] - hb);
     const int fb = td.face_off[t];
@@ -1185,11 +1251,151 @@ __global__ void __launch_bounds__(NTHR) k_grad_lsq_tile(Geom g, TileDev td, Stat
     }
 }
 
+template <int DIM, typename T, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_grad_lsq_tile(Geom g, TileDev td, State<T> s, BatchDev bd) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    for_each_tile<DIM + 2>(td, bd, nullptr, [&](int t, unsigned long long *) {
+        grad_lsq_tile_body<DIM, T>(g, td, s, t, smem);
+    });
+}
+
+
+// ---------------------------------------------------------------------------
+// Batch bookkeeping, one workgroup, run after every iteration of a batch:
+// per-instance residual max-norms from the atomic slots (which it clears),
+// the residual history row, the drop against the instance's reference row,
+// and the freeze of every instance that reached the tolerance (or went
+// non-finite).  When the active set changed it rebuilds the list of active
+// tiles, in instance order.  It also clears the queue heads of the next
+// iteration's launches.
+// ---------------------------------------------------------------------------
+struct BatchCheck {
+    int n, H, n_ctr;
+    unsigned long long *res;  // [n][WS][kResSlots]
+    double *res0;             // [n][WS]; negative: take the next row
+    double *hist;             // [H][n][WS]
+    double *drop;             // [n]
+    int *active, *conv, *frozen_buf, *iter;
+    const int *tile_begin;    // [n+1]
+    int *list, *count, *ctr;
+};
+
+constexpr int kCheckThreads = 1024;
+
+__device__ void rebuild_list(const BatchCheck &b) {
+    __shared__ int s_off[kCheckThreads];
+    __shared__ int s_tb[kCheckThreads], s_nt[kCheckThreads];
+    int carry = 0;
+    for (int base = 0; base < b.n; base += kCheckThreads) {
+        const int i = base + (int)threadIdx.x;
+        int nt = 0, tb = 0;
+        if (i < b.n) {
+            tb = b.tile_begin[i];
+            nt = b.active[i] ? b.tile_begin[i + 1] - tb : 0;
+        }
+        s_off[threadIdx.x] = nt;
+        __syncthreads();
+        for (int off = 1; off < kCheckThreads; off <<= 1) {  // inclusive scan
+            const int x = (int)threadIdx.x >= off ? s_off[threadIdx.x - off] : 0;
+            __syncthreads();
+            s_off[threadIdx.x] += x;
+            __syncthreads();
+        }
+        const int incl = s_off[threadIdx.x], total = s_off[kCheckThreads - 1];
+        __syncthreads();
+        s_off[threadIdx.x] = carry + incl - nt;
+        s_tb[threadIdx.x] = tb;
+        s_nt[threadIdx.x] = nt;
+        __syncthreads();
+        const int m = min(kCheckThreads, b.n - base);
+        for (int ii = 0; ii < m; ++ii) {
+            const int n_t = s_nt[ii], o = s_off[ii], t0 = s_tb[ii];
+            for (int k = threadIdx.x; k < n_t; k += kCheckThreads) b.list[o + k] = t0 + k;
+        }
+        carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *b.count = carry;
+}
+
+template <int WS>
+__global__ void __launch_bounds__(kCheckThreads) k_batch_check(BatchCheck b, double tol, int cur_after,
+                                                               int rebuild_only) {
+    __shared__ int s_changed;
+    if (threadIdx.x == 0) s_changed = rebuild_only;
+    __syncthreads();
+    const int it = *b.iter;
+    if (!rebuild_only) {
+        double *hrow = b.hist + (size_t)(it % b.H) * b.n * WS;
+        for (int i = threadIdx.x; i < b.n; i += kCheckThreads) {
+            if (!b.active[i]) {
+#pragma unroll
+                for (int j = 0; j < WS; ++j) hrow[(size_t)i * WS + j] = 0.0;
+                continue;
+            }
+            double r[WS];
+            bool bad = false;
+#pragma unroll
+            for (int j = 0; j < WS; ++j) {
+                unsigned long long *sl = b.res + ((size_t)i * WS + j) * kResSlots, mx = 0;
+                for (int k = 0; k < kResSlots; ++k) {
+                    const unsigned long long v = sl[k];
+                    mx = mx > v ? mx : v;
+                    sl[k] = 0;
+                }
+                r[j] = __longlong_as_double((long long)mx);
+                if (!(r[j] <= 1.0e300)) bad = true;
+                hrow[(size_t)i * WS + j] = r[j];
+            }
+            if (b.res0[(size_t)i * WS] < 0.0)
+#pragma unroll
+                for (int j = 0; j < WS; ++j) b.res0[(size_t)i * WS + j] = r[j];
+            double d = 0.0;
+#pragma unroll
+            for (int j = 0; j < WS; ++j) {
+                const double r0 = b.res0[(size_t)i * WS + j];
+                const double q = r0 > 0.0 ? r[j] / r0 : 0.0;
+                d = d > q ? d : q;
+            }
+            b.drop[i] = d;
+            if (bad || (tol > 0.0 && d <= tol)) {
+                b.active[i] = 0;
+                b.conv[i] = bad ? -2 : it;
+                b.frozen_buf[i] = cur_after;
+                s_changed = 1;
+            }
+        }
+    }
+    for (int k = threadIdx.x; k < 8 * b.n_ctr; k += kCheckThreads) b.ctr[k] = 0;
+    __syncthreads();
+    if (!rebuild_only && threadIdx.x == 0) *b.iter = it + 1;
+    if (s_changed) rebuild_list(b);
+}
 }  // namespace
 
 // ===========================================================================
 // Context
 // ===========================================================================
+// Batch of independent instances sharing one union mesh (psst_batch_*).
+struct BatchState {
+    int n = 0, ws = 0, H = 256;
+    int32_t N_tot = 0;
+    std::vector<int32_t> cell_begin, ghost_begin, tile_begin;  // [n+1] union ranges (device numbering)
+    std::vector<int32_t> c2d;                                  // union canonical -> device
+    std::vector<char> have_state;
+    std::vector<int32_t> buf_of;  // state buffer holding each instance's latest state
+    int *tile_inst = nullptr, *tile_begin_dev = nullptr, *list = nullptr, *count = nullptr, *ctr = nullptr;
+    unsigned long long *res = nullptr;
+    double *res0 = nullptr, *hist = nullptr, *drop = nullptr;
+    int *active = nullptr, *conv = nullptr, *frozen_buf = nullptr, *iter = nullptr;
+    int64_t iter_host = 0;
+    bool w_dirty = false;
+    bool use_graph = true;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    int graph_precision = 0;
+    double graph_tol = -1.0;
+};
+
 struct psst_ctx {
     int dim = 2, N = 0, G = 0, F = 0, NT = 0, nnz_visit = 0, nnz_list = 0;
     psst_settings s{};
@@ -1212,13 +1418,19 @@ struct psst_ctx {
     int max_level = 0;
     // residual normalisation for the tolerance test
     std::vector<double> res0;
+    char *snap = nullptr;  // state at the start of a chunk of a run with a tolerance
     // device numbering: d2c[d] = canonical index of device cell d (NT entries)
     std::vector<int32_t> d2c;
     // fused tiled path
     bool fused = false;
     TileDev td{};
-    int n_tiles = 0, tile_threads = 256;
+    int n_tiles = 0, tile_threads = 256, tile_cells = 0;
+    int max_slots = 0, max_faces = 0;
     size_t shm_stage = 0, shm_grad = 0;
+    // batched launches: per-launch work list (list == nullptr when unbatched)
+    BatchDev bd{};
+    int persist_blocks = 0;             // grid of a persistent batched launch
+    std::unique_ptr<struct BatchState> batch;
     // timing
     bool timing = false;
     struct KStat {
@@ -1348,47 +1560,89 @@ void launch_shm(psst_ctx *c, const char *name, K kernel, dim3 grid, dim3 block, 
     }
 }
 
+// Grid of a tile kernel: one workgroup per tile (8-aligned for the XCD
+// mapping), or, for a batched launch, a persistent grid sized to the
+// kernel's occupancy on every CU.
+template <typename K>
+dim3 tile_grid(psst_ctx *c, K kernel, int threads, size_t shm) {
+    if (!c->bd.list) return dim3(8 * ((c->n_tiles + 7) / 8));
+    static std::map<std::pair<const void *, size_t>, int> occ;
+    auto key = std::make_pair((const void *)kernel, shm);
+    auto it = occ.find(key);
+    if (it == occ.end()) {
+        set_shm(kernel, shm);
+        int per_cu = 0, dev = 0, n_cu = 0;
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, shm), "occupancy");
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+        it = occ.emplace(key, std::max(1, per_cu) * std::max(1, n_cu)).first;
+    }
+    const int g = std::min(it->second, c->n_tiles);
+    return dim3(8 * ((g + 7) / 8));
+}
+
+// Work list of launch `idx` of an iteration (0: gradients, 1 + k: stage k):
+// each launch has its own queue heads.
+BatchDev launch_bd(psst_ctx *c, int idx) {
+    BatchDev b = c->bd;
+    if (b.list) b.ctr = c->bd.ctr + 8 * idx;
+    return b;
+}
+
+size_t stage_shm(const psst_ctx *c, size_t sb) {
+    const int NQ = c->dim + 3, NV = c->dim + 2;
+    return ((size_t)NQ * c->max_slots + (size_t)NV * c->max_faces) * sb + (size_t)c->max_slots * sizeof(int);
+}
+size_t grad_shm(const psst_ctx *c, size_t sb) { return (size_t)(c->dim + 2) * c->max_slots * sb; }
+
+template <int DIM, typename T, int NTHR>
+void run_grad_tile(psst_ctx *c) {
+    State<T> st = state<T>(c);
+    const size_t shm = grad_shm(c, sizeof(T));
+    const BatchDev bd = launch_bd(c, 0);
+    if (c->s.grad == PSST_GRAD_GREEN_GAUSS) {
+        auto k = k_grad_gg_tile<DIM, T, NTHR>;
+        launch_shm(c, "grad_gg_tile", k, tile_grid(c, k, NTHR, shm), dim3(NTHR), shm, c->geom, c->td, st, bd);
+    } else {
+        auto k = k_grad_lsq_tile<DIM, T, NTHR>;
+        launch_shm(c, "grad_lsq_tile", k, tile_grid(c, k, NTHR, shm), dim3(NTHR), shm, c->geom, c->td, st, bd);
+    }
+}
+
 template <int DIM, typename T>
 void run_grad(psst_ctx *c) {
-    State<T> st = state<T>(c);
     if (c->fused) {
-        const bool big = c->tile_threads == 512;
-        dim3 gr(8 * ((c->n_tiles + 7) / 8)), bl(c->tile_threads);
-        if (c->s.grad == PSST_GRAD_GREEN_GAUSS) {
-            if (big) launch_shm(c, "grad_gg_tile", k_grad_gg_tile<DIM, T, 512>, gr, bl, c->shm_grad, c->geom, c->td, st);
-            else launch_shm(c, "grad_gg_tile", k_grad_gg_tile<DIM, T, 256>, gr, bl, c->shm_grad, c->geom, c->td, st);
-        } else {
-            if (big) launch_shm(c, "grad_lsq_tile", k_grad_lsq_tile<DIM, T, 512>, gr, bl, c->shm_grad, c->geom, c->td, st);
-            else launch_shm(c, "grad_lsq_tile", k_grad_lsq_tile<DIM, T, 256>, gr, bl, c->shm_grad, c->geom, c->td, st);
-        }
+        if (c->tile_threads == 512) run_grad_tile<DIM, T, 512>(c);
+        else run_grad_tile<DIM, T, 256>(c);
         return;
     }
+    State<T> st = state<T>(c);
     if (c->s.grad == PSST_GRAD_GREEN_GAUSS)
         launch(c, "grad_gg", k_grad_gg<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st);
     else
         launch(c, "grad_lsq", k_grad_lsq<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st);
 }
 
-template <int DIM, typename T, int FLUX, int ORDER, int LIM, int NTHR>
-void run_stage_lc_t(psst_ctx *c, int k, unsigned long long *res_row) {
+template <int DIM, typename T, int FLUX, int ORDER, int LIM, int LC, int NTHR>
+void run_stage_k(psst_ctx *c, int k, unsigned long long *res_row) {
     State<T> st = state<T>(c);
     const T alpha = T(c->s.rk_coeff[k]);
     const int last = (k == c->s.rk_levels - 1);
-    const char *nm = "stage";
-    dim3 gr(8 * ((c->n_tiles + 7) / 8)), bl(NTHR);
+    const size_t shm = stage_shm(c, sizeof(T));
+    auto kern = k_stage<DIM, T, FLUX, ORDER, LIM, LC, NTHR>;
+    launch_shm(c, "stage", kern, tile_grid(c, kern, NTHR, shm), dim3(NTHR), shm, c->geom, c->td, st, c->ph, alpha,
+               last, res_row, (const BCDev *)c->bc, launch_bd(c, 1 + k));
+}
+
+template <int DIM, typename T, int FLUX, int ORDER, int LIM, int NTHR>
+void run_stage_lc_t(psst_ctx *c, int k, unsigned long long *res_row) {
     switch (c->s.time_mode) {
     case PSST_DT_UNIFORM:
-        launch_shm(c, nm, k_stage<DIM, T, FLUX, ORDER, LIM, PSST_DT_UNIFORM, NTHR>, gr, bl, c->shm_stage, c->geom,
-                   c->td, st, c->ph, alpha, last, res_row, (const BCDev *)c->bc);
-        break;
+        return run_stage_k<DIM, T, FLUX, ORDER, LIM, PSST_DT_UNIFORM, NTHR>(c, k, res_row);
     case PSST_DT_LOCAL1:
-        launch_shm(c, nm, k_stage<DIM, T, FLUX, ORDER, LIM, PSST_DT_LOCAL1, NTHR>, gr, bl, c->shm_stage, c->geom,
-                   c->td, st, c->ph, alpha, last, res_row, (const BCDev *)c->bc);
-        break;
+        return run_stage_k<DIM, T, FLUX, ORDER, LIM, PSST_DT_LOCAL1, NTHR>(c, k, res_row);
     default:
-        launch_shm(c, nm, k_stage<DIM, T, FLUX, ORDER, LIM, PSST_DT_LOCAL2, NTHR>, gr, bl, c->shm_stage, c->geom,
-                   c->td, st, c->ph, alpha, last, res_row, (const BCDev *)c->bc);
-        break;
+        return run_stage_k<DIM, T, FLUX, ORDER, LIM, PSST_DT_LOCAL2, NTHR>(c, k, res_row);
     }
 }
 
@@ -1583,7 +1837,12 @@ extern "C" {
 
 const char *psst_last_error(const psst_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
 
-psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, psst_ctx **out) {
+}  // extern "C"
+
+// Context creation; seg (may be empty) lists the first interior cell of each
+// independent instance of a batch.
+static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, const std::vector<int32_t> &seg,
+                               psst_ctx **out) {
     if (!m || !s || !out) {
         g_create_error = "null argument";
         return PSST_BAD_ARG;
@@ -1651,7 +1910,7 @@ psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, pss
                 throw std::invalid_argument("unsupported thermal boundary kind");
         }
         // ---- device numbering: interior cells along a Morton curve, ghosts unchanged ----
-        std::vector<int32_t> d2c_int = psst::morton_order(dim, N, NT, m->cell_centre);
+        std::vector<int32_t> d2c_int = psst::morton_order(dim, N, NT, m->cell_centre, seg);
         c->d2c.resize((size_t)NT);
         std::vector<int32_t> c2d((size_t)NT);
         for (int d = 0; d < N; ++d) c->d2c[(size_t)d] = d2c_int[(size_t)d];
@@ -1830,12 +2089,12 @@ psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, pss
             if (s->tile_threads != 0 && s->tile_threads != 256 && s->tile_threads != 512)
                 throw std::invalid_argument("tile_threads must be 256 or 512");
             c->tile_threads = s->tile_threads ? s->tile_threads : (tile >= 512 ? 512 : 256);
+            c->tile_cells = tile;
             psst::TilePlan tp = psst::plan_tiles(N, G, F, tile, f0d.data(), f1d.data(), voff.data(), vface.data(),
-                                                 loff.data(), lface.data());
-            const int NQ = dim + 3, NV = dim + 2;
-            const size_t shm_stage =
-                ((size_t)NQ * tp.max_slots + (size_t)NV * tp.max_faces) * sb + (size_t)tp.max_slots * sizeof(int);
-            const size_t shm_grad = (size_t)NV * tp.max_slots * sb;
+                                                 loff.data(), lface.data(), seg);
+            c->max_slots = tp.max_slots;
+            c->max_faces = tp.max_faces;
+            const size_t shm_stage = stage_shm(c.get(), sb), shm_grad = grad_shm(c.get(), sb);
             if (shm_stage <= 160 * 1024) {
                 const int TF = (int)tp.face_id.size();
                 std::vector<double> tfn((size_t)dim * TF), tfa((size_t)TF), trlr((size_t)dim * TF), tbeta((size_t)TF);
@@ -1882,21 +2141,23 @@ psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, pss
                 c->fused = true;
             }
         }
-        // ---- state ----
+        // ---- state: sized for FP64 whatever the precision, so that
+        // psst_set_precision can switch in place ----
+        const size_t sb8 = sizeof(double);
         for (int i = 0; i < 2; ++i) {
-            hip_check(hipMalloc(&c->q[i], sb * (size_t)(dim + 3) * NT), "hipMalloc");
+            hip_check(hipMalloc(&c->q[i], sb8 * (size_t)(dim + 3) * NT), "hipMalloc");
             c->allocs.push_back(c->q[i]);
         }
-        hip_check(hipMalloc(&c->W0, sb * (size_t)(dim + 2) * N), "hipMalloc");
+        hip_check(hipMalloc(&c->W0, sb8 * (size_t)(dim + 2) * N), "hipMalloc");
         c->allocs.push_back(c->W0);
-        hip_check(hipMalloc(&c->grad, sb * (size_t)(dim == 2 ? GradLayout<2>::STRIDE : GradLayout<3>::STRIDE) * N),
+        hip_check(hipMalloc(&c->grad, sb8 * (size_t)(dim == 2 ? GradLayout<2>::STRIDE : GradLayout<3>::STRIDE) * N),
                   "hipMalloc");
         c->allocs.push_back(c->grad);
-        hip_check(hipMalloc(&c->FA, sb * (size_t)(dim + 2) * F), "hipMalloc");
+        hip_check(hipMalloc(&c->FA, sb8 * (size_t)(dim + 2) * F), "hipMalloc");
         c->allocs.push_back(c->FA);
         ensure_res(c.get(), 64);
-        c->kernel_names = {"grad_gg",      "grad_lsq",   "flux", "update", "update_level", "update_levels_block",
-                           "bc",           "stage",      "grad_gg_tile", "grad_lsq_tile"};
+        c->kernel_names = {"grad_gg", "grad_lsq", "flux",         "update",        "update_level", "update_levels_block",
+                           "bc",      "stage",    "grad_gg_tile", "grad_lsq_tile", "batch_check"};
     } catch (const std::exception &e) {
         g_create_error = e.what();
         for (void *p : c->allocs) (void)hipFree(p);
@@ -1905,6 +2166,12 @@ psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, pss
     }
     *out = c.release();
     return PSST_OK;
+}
+
+extern "C" {
+
+psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, psst_ctx **out) {
+    return create_impl(m, s, {}, out);
 }
 
 psst_status psst_set_state(psst_ctx *c, const double *prim) {
@@ -1957,26 +2224,44 @@ psst_status psst_run(psst_ctx *c, int32_t n_iter, double tol, double *res_hist, 
     if (n_iter <= 0) return PSST_OK;
     const int ws = c->dim + 2;
     try {
-        // residual rows are produced in chunks so a tolerance can stop the run
-        const int chunk = tol > 0.0 ? std::min<int32_t>(n_iter, 50) : n_iter;
+        // Residual rows come back once per chunk.  With a tolerance, the state
+        // at the start of each chunk is kept; if the tolerance is met inside a
+        // chunk, the state is restored and exactly the iterations up to that
+        // one are replayed (the solver is deterministic, so the replay is
+        // bitwise the same), and the run ends on that iteration's state.
+        const int chunk = tol > 0.0 ? std::min<int32_t>(n_iter, 64) : n_iter;
         ensure_res(c, chunk);
         std::vector<unsigned long long> slots((size_t)chunk * ws * kResSlots);
-        int done = 0;
-        bool stop = false;
-        bool bad = false;
-        while (done < n_iter && !stop) {
-            const int n = std::min(chunk, n_iter - done);
+        const size_t sb = state_bytes(c);
+        const size_t nq = (size_t)(c->dim + 3) * c->NT * sb, nw = (size_t)(c->dim + 2) * c->N * sb;
+        if (tol > 0.0 && !c->snap)  // sized for FP64, whatever the current precision
+            c->snap = dev_alloc<char>(c, ((size_t)(c->dim + 3) * c->NT + (size_t)(c->dim + 2) * c->N) * sizeof(double));
+        auto run_iters = [&](int n) {
             const size_t nres = (size_t)n * ws * kResSlots;
             launch(c, "fill", k_fill_u64, dim3(blocks((long)nres)), dim3(kBlock), c->res, nres, 0ull);
             for (int it = 0; it < n; ++it) {
                 unsigned long long *row = c->res + (size_t)it * ws * kResSlots;
                 c->s.precision == 64 ? dispatch_iteration<double>(c, row) : dispatch_iteration<float>(c, row);
             }
+        };
+        int done = 0;
+        bool stop = false;
+        bool bad = false;
+        while (done < n_iter && !stop) {
+            const int n = std::min(chunk, n_iter - done);
+            const size_t nres = (size_t)n * ws * kResSlots;
+            const int cur0 = c->cur;
+            if (tol > 0.0) {
+                hip_check(hipMemcpyAsync(c->snap, c->q[cur0], nq, hipMemcpyDeviceToDevice, c->stream), "snapshot");
+                hip_check(hipMemcpyAsync(c->snap + nq, c->W0, nw, hipMemcpyDeviceToDevice, c->stream), "snapshot");
+            }
+            run_iters(n);
             hip_check(hipMemcpyAsync(slots.data(), c->res, nres * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                      c->stream),
                       "hipMemcpy D2H");
             hip_check(hipStreamSynchronize(c->stream), "psst_run");
             if (c->timing) flush_timing(c);
+            int stop_at = -1;
             for (int it = 0; it < n; ++it) {
                 double row[5];
                 for (int j = 0; j < ws; ++j) {
@@ -1991,17 +2276,32 @@ psst_status psst_run(psst_ctx *c, int32_t n_iter, double tol, double *res_hist, 
                 if (res_hist)
                     for (int j = 0; j < ws; ++j) res_hist[(size_t)(done + it) * ws + j] = row[j];
                 if (c->res0.empty()) c->res0.assign(row, row + ws);
-                if (tol > 0.0 && !stop) {
+                if (tol > 0.0) {
                     double drop = 0.0;
                     for (int j = 0; j < ws; ++j) drop = std::max(drop, c->res0[(size_t)j] > 0 ? row[j] / c->res0[(size_t)j] : 0.0);
                     if (drop <= tol) {
-                        stop = true;
-                        if (iters_done) *iters_done = done + it + 1;
+                        stop_at = it;
+                        break;
                     }
                 }
             }
+            if (stop_at >= 0) {
+                stop = true;
+                if (stop_at < n - 1) {
+                    // restore the chunk's start and replay up to the stopping iteration
+                    c->cur = cur0;
+                    hip_check(hipMemcpyAsync(c->q[cur0], c->snap, nq, hipMemcpyDeviceToDevice, c->stream), "restore");
+                    hip_check(hipMemcpyAsync(c->W0, c->snap + nq, nw, hipMemcpyDeviceToDevice, c->stream), "restore");
+                    run_iters(stop_at + 1);
+                    hip_check(hipStreamSynchronize(c->stream), "psst_run replay");
+                    if (c->timing) flush_timing(c);
+                }
+                done += stop_at + 1;
+                if (iters_done) *iters_done = done;
+                break;
+            }
             done += n;
-            if (!stop && iters_done) *iters_done = done;
+            if (iters_done) *iters_done = done;
         }
         if (bad) {
             c->err = "non-finite residual: the state contains NaN/inf (a negative temperature or density upstream)";
@@ -2175,6 +2475,9 @@ psst_status psst_copy_calibration(int64_t bytes, int32_t reps, double *mean_ms) 
 void psst_ctx_destroy(psst_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->batch)
+        for (hipGraphExec_t &e : c->batch->exec)
+            if (e) (void)hipGraphExecDestroy(e);
     for (auto &p : c->pending) {
         (void)hipEventDestroy(p.second.first);
         (void)hipEventDestroy(p.second.second);
@@ -2183,6 +2486,637 @@ void psst_ctx_destroy(psst_ctx *c) {
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Precision switch, batches of instances, autotuner
+// ===========================================================================
+namespace {
+
+void destroy_graphs(psst_ctx *c) {
+    if (!c->batch) return;
+    for (hipGraphExec_t &e : c->batch->exec) {
+        if (e) (void)hipGraphExecDestroy(e);
+        e = nullptr;
+    }
+}
+
+BatchCheck batch_check_args(psst_ctx *c) {
+    BatchState &b = *c->batch;
+    BatchCheck k;
+    k.n = b.n;
+    k.H = b.H;
+    k.n_ctr = 1 + c->s.rk_levels;
+    k.res = b.res;
+    k.res0 = b.res0;
+    k.hist = b.hist;
+    k.drop = b.drop;
+    k.active = b.active;
+    k.conv = b.conv;
+    k.frozen_buf = b.frozen_buf;
+    k.iter = b.iter;
+    k.tile_begin = b.tile_begin_dev;
+    k.list = b.list;
+    k.count = b.count;
+    k.ctr = b.ctr;
+    return k;
+}
+
+void launch_check(psst_ctx *c, double tol, int cur_after, int rebuild_only) {
+    const BatchCheck k = batch_check_args(c);
+    if (c->dim == 2)
+        launch(c, "batch_check", k_batch_check<4>, dim3(1), dim3(kCheckThreads), k, tol, cur_after, rebuild_only);
+    else
+        launch(c, "batch_check", k_batch_check<5>, dim3(1), dim3(kCheckThreads), k, tol, cur_after, rebuild_only);
+}
+
+// One batch iteration: gradients, the RK stages over the active tiles, and
+// the per-instance bookkeeping.
+void batch_iteration(psst_ctx *c, double tol) {
+    c->s.precision == 64 ? dispatch_iteration<double>(c, nullptr) : dispatch_iteration<float>(c, nullptr);
+    launch_check(c, tol, c->cur, 0);
+}
+
+// The same iteration as a captured graph (one per parity of the state
+// buffers at its start: with an odd stage count the buffers alternate).
+void batch_graph_iteration(psst_ctx *c, double tol) {
+    BatchState &b = *c->batch;
+    if (b.graph_precision != c->s.precision || b.graph_tol != tol) {
+        destroy_graphs(c);
+        b.graph_precision = c->s.precision;
+        b.graph_tol = tol;
+    }
+    const int parity = c->cur;
+    hipGraphExec_t &ex = b.exec[parity];
+    if (!ex) {
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+        try {
+            batch_iteration(c, tol);
+        } catch (...) {
+            (void)hipStreamEndCapture(c->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            c->cur = parity;
+            throw;
+        }
+        hip_check(hipStreamEndCapture(c->stream, &g), "hipStreamEndCapture");
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) ex = nullptr;
+        hip_check(e, "hipGraphInstantiate");
+        c->cur = parity;
+    }
+    hip_check(hipGraphLaunch(ex, c->stream), "hipGraphLaunch");
+    c->cur = parity ^ (c->s.rk_levels & 1);
+}
+
+// Brings every instance's latest state into the current state buffer.
+void batch_gather_states(psst_ctx *c) {
+    BatchState &b = *c->batch;
+    const size_t sb = state_bytes(c);
+    for (int i = 0; i < b.n; ++i) {
+        const int from = b.buf_of[(size_t)i];
+        if (from == c->cur) continue;
+        const char *src = static_cast<const char *>(c->q[from]);
+        char *dst = static_cast<char *>(c->q[c->cur]);
+        for (int v = 0; v < c->dim + 3; ++v) {
+            const size_t ci = (size_t)v * c->NT + b.cell_begin[(size_t)i];
+            const size_t nc = (size_t)(b.cell_begin[(size_t)i + 1] - b.cell_begin[(size_t)i]);
+            hip_check(hipMemcpyAsync(dst + ci * sb, src + ci * sb, nc * sb, hipMemcpyDeviceToDevice, c->stream), "D2D");
+            const size_t gi = (size_t)v * c->NT + b.N_tot + b.ghost_begin[(size_t)i];
+            const size_t ng = (size_t)(b.ghost_begin[(size_t)i + 1] - b.ghost_begin[(size_t)i]);
+            if (ng)
+                hip_check(hipMemcpyAsync(dst + gi * sb, src + gi * sb, ng * sb, hipMemcpyDeviceToDevice, c->stream),
+                          "D2D");
+        }
+        b.buf_of[(size_t)i] = c->cur;
+    }
+}
+
+void upload_values(psst_ctx *c, void *dev_base, size_t offset, const std::vector<double> &vals) {
+    if (vals.empty()) return;
+    if (c->s.precision == 64) {
+        hip_check(hipMemcpy(static_cast<double *>(dev_base) + offset, vals.data(), vals.size() * sizeof(double),
+                            hipMemcpyHostToDevice),
+                  "hipMemcpy");
+    } else {
+        std::vector<float> f(vals.begin(), vals.end());
+        hip_check(hipMemcpy(static_cast<float *>(dev_base) + offset, f.data(), f.size() * sizeof(float),
+                            hipMemcpyHostToDevice),
+                  "hipMemcpy");
+    }
+}
+
+std::vector<double> download_values(psst_ctx *c, const void *dev_base, size_t offset, size_t n) {
+    std::vector<double> out(n);
+    if (!n) return out;
+    if (c->s.precision == 64) {
+        hip_check(hipMemcpy(out.data(), static_cast<const double *>(dev_base) + offset, n * sizeof(double),
+                            hipMemcpyDeviceToHost),
+                  "hipMemcpy");
+    } else {
+        std::vector<float> f(n);
+        hip_check(hipMemcpy(f.data(), static_cast<const float *>(dev_base) + offset, n * sizeof(float),
+                            hipMemcpyDeviceToHost),
+                  "hipMemcpy");
+        for (size_t k = 0; k < n; ++k) out[k] = f[k];
+    }
+    return out;
+}
+
+bool same_scheme(const psst_settings &a, const psst_settings &b) {
+    bool eq = a.flux == b.flux && a.order == b.order && a.grad == b.grad && a.limiter == b.limiter &&
+              a.time_mode == b.time_mode && a.rk_levels == b.rk_levels && a.cfl == b.cfl && a.gamma == b.gamma &&
+              a.rm == b.rm && a.cp == b.cp && a.cv == b.cv && a.precision == b.precision &&
+              a.dt_ordering == b.dt_ordering && a.venkat_k == b.venkat_k && a.tile_cells == b.tile_cells &&
+              a.tile_threads == b.tile_threads;
+    for (int k = 0; k < a.rk_levels && k < 8; ++k) eq = eq && a.rk_coeff[k] == b.rk_coeff[k];
+    if (a.time_mode == PSST_DT_UNIFORM) eq = eq && a.uniform_dt == b.uniform_dt;
+    return eq;
+}
+
+}  // namespace
+
+struct psst_batch {
+    psst_ctx *ctx = nullptr;
+    std::string err;
+    ~psst_batch() {
+        if (ctx) psst_ctx_destroy(ctx);
+    }
+};
+
+extern "C" {
+
+psst_status psst_set_precision(psst_ctx *c, int32_t precision) {
+    if (!c) return PSST_BAD_ARG;
+    if (precision != 32 && precision != 64) {
+        c->err = "precision must be 32 or 64";
+        return PSST_BAD_ARG;
+    }
+    if (precision == c->s.precision) return PSST_OK;
+    try {
+        const size_t nsb = precision == 64 ? sizeof(double) : sizeof(float);
+        if (c->fused && stage_shm(c, nsb) > 160 * 1024) {
+            c->err = "the tile plan does not fit in LDS at this precision";
+            return PSST_UNSUPPORTED;
+        }
+        const size_t nq = (size_t)(c->dim + 3) * c->NT, nw = (size_t)(c->dim + 2) * c->N;
+        void *tmp = nullptr;
+        hip_check(hipMalloc(&tmp, nq * sizeof(double)), "hipMalloc");
+        try {
+            void *bufs[3] = {c->q[0], c->q[1], c->W0};
+            const size_t cnt[3] = {nq, nq, nw};
+            for (int k = 0; k < 3; ++k) {
+                if (precision == 64) {
+                    hipLaunchKernelGGL((k_convert<float, double>), dim3(blocks((long)cnt[k])), dim3(kBlock), 0,
+                                       c->stream, (const float *)bufs[k], (double *)tmp, cnt[k]);
+                    hip_check(hipGetLastError(), "convert");
+                    hip_check(hipMemcpyAsync(bufs[k], tmp, cnt[k] * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
+                              "D2D");
+                } else {
+                    hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks((long)cnt[k])), dim3(kBlock), 0,
+                                       c->stream, (const double *)bufs[k], (float *)tmp, cnt[k]);
+                    hip_check(hipGetLastError(), "convert");
+                    hip_check(hipMemcpyAsync(bufs[k], tmp, cnt[k] * sizeof(float), hipMemcpyDeviceToDevice, c->stream),
+                              "D2D");
+                }
+            }
+            hip_check(hipStreamSynchronize(c->stream), "convert");
+        } catch (...) {
+            (void)hipFree(tmp);
+            throw;
+        }
+        hip_check(hipFree(tmp), "hipFree");
+        c->s.precision = precision;
+        c->shm_stage = c->fused ? stage_shm(c, nsb) : 0;
+        c->shm_grad = c->fused ? grad_shm(c, nsb) : 0;
+        destroy_graphs(c);
+        return PSST_OK;
+    } catch (const std::exception &e) {
+        return set_error(c, e, PSST_HIP_ERROR);
+    }
+}
+
+const char *psst_batch_last_error(const psst_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+psst_status psst_batch_create(int32_t n, const psst_mesh_desc *meshes, const psst_settings *settings,
+                              psst_batch **out) {
+    if (n <= 0 || !meshes || !settings || !out) {
+        g_create_error = "psst_batch_create: null argument or empty batch";
+        return PSST_BAD_ARG;
+    }
+    *out = nullptr;
+    try {
+        const int dim = meshes[0].dim;
+        const psst_settings &s0 = settings[0];
+        if (s0.time_mode == PSST_DT_LOCAL1 && s0.dt_ordering == 0)
+            throw std::invalid_argument("batches need an order-free time step (time_mode 0 or 2, or dt_ordering 1)");
+        int64_t N = 0, G = 0, F = 0, nv = 0, nl = 0, nbc = 0;
+        bool lsq = s0.order == 2 && s0.grad == PSST_GRAD_LEAST_SQUARES;
+        for (int i = 0; i < n; ++i) {
+            const psst_mesh_desc &m = meshes[i];
+            if (m.dim != dim) throw std::invalid_argument("instances of a batch must share the dimension");
+            if (!same_scheme(settings[i], s0)) throw std::invalid_argument("instances of a batch must share the scheme");
+            if (m.n_cells <= 0 || m.n_faces <= 0 || m.n_ghosts < 0) throw std::invalid_argument("empty instance mesh");
+            if (settings[i].n_bc <= 0 || !settings[i].bc) throw std::invalid_argument("boundary table missing");
+            N += m.n_cells;
+            G += m.n_ghosts;
+            F += m.n_faces;
+            nv += m.visit_offsets[m.n_cells];
+            nl += m.list_offsets[m.n_cells];
+            nbc += settings[i].n_bc;
+        }
+        const int64_t NT = N + G;
+        if (NT >= INT32_MAX / 2 || F >= INT32_MAX / 2 || nv >= INT32_MAX / 2 || nl >= INT32_MAX / 2)
+            throw std::invalid_argument("batch too large for 32-bit indices");
+        std::vector<double> vol((size_t)NT), cen((size_t)dim * NT), nrm((size_t)dim * F), area((size_t)F),
+            beta((size_t)F), lsq_r(lsq ? 6 * (size_t)N : 0), lsq_b(lsq ? (size_t)N : 0);
+        std::vector<int32_t> fc(2 * (size_t)F), voff((size_t)N + 1), vf((size_t)nv), loff((size_t)N + 1), lf((size_t)nl),
+            gface((size_t)G), gbc((size_t)G), seg((size_t)n);
+        std::vector<psst_bc> bcs;
+        bcs.reserve((size_t)nbc);
+        auto bs = std::make_unique<BatchState>();
+        bs->n = n;
+        bs->ws = dim + 2;
+        bs->N_tot = (int32_t)N;
+        bs->cell_begin.assign(1, 0);
+        bs->ghost_begin.assign(1, 0);
+        int64_t cb = 0, gb = 0, fb = 0, vb = 0, lb = 0, bb = 0;
+        voff[0] = loff[0] = 0;
+        for (int i = 0; i < n; ++i) {
+            const psst_mesh_desc &m = meshes[i];
+            const int64_t Ni = m.n_cells, Gi = m.n_ghosts, Fi = m.n_faces, NTi = Ni + Gi;
+            auto uc = [&](int32_t x) -> int32_t {  // instance cell -> union canonical cell
+                return (int32_t)(x < Ni ? cb + x : N + gb + (x - Ni));
+            };
+            seg[(size_t)i] = (int32_t)cb;
+            for (int64_t x = 0; x < NTi; ++x) {
+                const int32_t u = uc((int32_t)x);
+                vol[(size_t)u] = m.cell_volume[x];
+                for (int k = 0; k < dim; ++k) cen[(size_t)k * NT + u] = m.cell_centre[(size_t)k * NTi + x];
+            }
+            for (int64_t f = 0; f < Fi; ++f) {
+                for (int k = 0; k < dim; ++k) nrm[(size_t)k * F + fb + f] = m.face_normal[(size_t)k * Fi + f];
+                area[(size_t)(fb + f)] = m.face_area[f];
+                beta[(size_t)(fb + f)] = m.face_beta[f];
+                const int32_t a = m.face_cells[f], b = m.face_cells[Fi + f];
+                if (a < 0 || a >= NTi || b < 0 || b >= NTi) throw std::invalid_argument("face_cells out of range");
+                fc[(size_t)(fb + f)] = uc(a);
+                fc[(size_t)(F + fb + f)] = uc(b);
+            }
+            for (int64_t x = 0; x < Ni; ++x) {
+                for (int32_t k = m.visit_offsets[x]; k < m.visit_offsets[x + 1]; ++k)
+                    vf[(size_t)(vb + k)] = (int32_t)(fb + m.visit_faces[k]);
+                for (int32_t k = m.list_offsets[x]; k < m.list_offsets[x + 1]; ++k)
+                    lf[(size_t)(lb + k)] = (int32_t)(fb + m.list_faces[k]);
+                voff[(size_t)(cb + x + 1)] = (int32_t)(vb + m.visit_offsets[x + 1]);
+                loff[(size_t)(cb + x + 1)] = (int32_t)(lb + m.list_offsets[x + 1]);
+                if (lsq) {
+                    if (!m.lsq_r || !m.lsq_beta) throw std::invalid_argument("least squares needs lsq_r and lsq_beta");
+                    for (int r = 0; r < 6; ++r) lsq_r[(size_t)r * N + cb + x] = m.lsq_r[(size_t)r * Ni + x];
+                    lsq_b[(size_t)(cb + x)] = m.lsq_beta[x];
+                }
+            }
+            for (int64_t g = 0; g < Gi; ++g) {
+                gface[(size_t)(gb + g)] = (int32_t)(fb + m.ghost_face[g]);
+                if (m.ghost_bc[g] < 0 || m.ghost_bc[g] >= settings[i].n_bc) throw std::invalid_argument("ghost_bc out of range");
+                gbc[(size_t)(gb + g)] = (int32_t)(bb + m.ghost_bc[g]);
+            }
+            for (int k = 0; k < settings[i].n_bc; ++k) bcs.push_back(settings[i].bc[k]);
+            cb += Ni;
+            gb += Gi;
+            fb += Fi;
+            vb += m.visit_offsets[Ni];
+            lb += m.list_offsets[Ni];
+            bb += settings[i].n_bc;
+            bs->cell_begin.push_back((int32_t)cb);
+            bs->ghost_begin.push_back((int32_t)gb);
+        }
+        double vmin = meshes[0].min_volume;
+        for (int i = 1; i < n; ++i) vmin = std::min(vmin, meshes[i].min_volume);
+        psst_mesh_desc um{};
+        um.dim = dim;
+        um.n_cells = (int32_t)N;
+        um.n_ghosts = (int32_t)G;
+        um.n_faces = (int32_t)F;
+        um.cell_volume = vol.data();
+        um.cell_centre = cen.data();
+        um.face_normal = nrm.data();
+        um.face_area = area.data();
+        um.face_cells = fc.data();
+        um.face_beta = beta.data();
+        um.visit_offsets = voff.data();
+        um.visit_faces = vf.data();
+        um.list_offsets = loff.data();
+        um.list_faces = lf.data();
+        um.ghost_face = gface.data();
+        um.ghost_bc = gbc.data();
+        um.lsq_r = lsq ? lsq_r.data() : nullptr;
+        um.lsq_beta = lsq ? lsq_b.data() : nullptr;
+        um.min_volume = vmin;
+        psst_settings us = s0;
+        us.fused = 1;
+        us.bc = bcs.data();
+        us.n_bc = (int32_t)bcs.size();
+        psst_ctx *c = nullptr;
+        psst_status st = create_impl(&um, &us, seg, &c);
+        if (st != PSST_OK) return st;
+        auto bt = std::make_unique<psst_batch>();
+        bt->ctx = c;
+        if (!c->fused) {
+            psst_ctx_destroy(c);
+            bt->ctx = nullptr;
+            throw std::invalid_argument("the batch's tiles do not fit in LDS (use smaller tile_cells)");
+        }
+        // tiles restart at every instance: ceil(N_i / tile) each, in order
+        bs->tile_begin.assign(1, 0);
+        std::vector<int32_t> tile_inst;
+        for (int i = 0; i < n; ++i) {
+            const int32_t Ni = meshes[i].n_cells;
+            const int32_t nt = (Ni + c->tile_cells - 1) / c->tile_cells;
+            for (int k = 0; k < nt; ++k) tile_inst.push_back(i);
+            bs->tile_begin.push_back(bs->tile_begin.back() + nt);
+        }
+        if (bs->tile_begin.back() != c->n_tiles) throw std::logic_error("batch tile count mismatch");
+        bs->c2d.resize((size_t)NT);
+        for (int64_t d = 0; d < NT; ++d) bs->c2d[(size_t)c->d2c[(size_t)d]] = (int32_t)d;
+        bs->have_state.assign((size_t)n, 0);
+        bs->buf_of.assign((size_t)n, 0);
+        const int ws = dim + 2;
+        psst_ctx *cx = c;
+        bs->tile_inst = dev_upload(cx, tile_inst.data(), tile_inst.size());
+        bs->tile_begin_dev = dev_upload(cx, bs->tile_begin.data(), bs->tile_begin.size());
+        bs->list = dev_alloc<int>(cx, (size_t)c->n_tiles);
+        std::vector<int> zeros_i((size_t)std::max(n, 8 * 9), 0), minus1((size_t)n, -1);
+        bs->count = dev_upload(cx, zeros_i.data(), 1);
+        bs->ctr = dev_upload(cx, zeros_i.data(), (size_t)8 * 9);
+        bs->iter = dev_upload(cx, zeros_i.data(), 1);
+        bs->active = dev_upload(cx, zeros_i.data(), (size_t)n);
+        bs->frozen_buf = dev_upload(cx, zeros_i.data(), (size_t)n);
+        bs->conv = dev_upload(cx, minus1.data(), (size_t)n);
+        std::vector<unsigned long long> zres((size_t)n * ws * kResSlots, 0ull);
+        bs->res = dev_upload(cx, zres.data(), zres.size());
+        std::vector<double> mres((size_t)n * ws, -1.0), zd((size_t)n, 0.0);
+        bs->res0 = dev_upload(cx, mres.data(), mres.size());
+        bs->drop = dev_upload(cx, zd.data(), zd.size());
+        bs->hist = dev_alloc<double>(cx, (size_t)bs->H * n * ws);
+        c->batch = std::move(bs);
+        *out = bt.release();
+        return PSST_OK;
+    } catch (const std::exception &e) {
+        g_create_error = e.what();
+        return dynamic_cast<const HipError *>(&e) ? PSST_HIP_ERROR : PSST_BAD_ARG;
+    }
+}
+
+psst_ctx *psst_batch_context(psst_batch *b) { return b ? b->ctx : nullptr; }
+
+psst_status psst_batch_set_state(psst_batch *bt, int32_t inst, const double *prim) {
+    if (!bt || !prim) return PSST_BAD_ARG;
+    psst_ctx *c = bt->ctx;
+    BatchState &b = *c->batch;
+    if (inst < 0 || inst >= b.n) {
+        bt->err = "instance index out of range";
+        return PSST_BAD_ARG;
+    }
+    try {
+        const int32_t cb = b.cell_begin[(size_t)inst], Ni = b.cell_begin[(size_t)inst + 1] - cb;
+        const int32_t gb = b.ghost_begin[(size_t)inst], Gi = b.ghost_begin[(size_t)inst + 1] - gb;
+        const size_t NTi = (size_t)Ni + Gi;
+        std::vector<double> vi((size_t)Ni), vg((size_t)Gi);
+        for (int v = 0; v < c->dim + 3; ++v) {
+            for (int32_t x = 0; x < Ni; ++x) vi[(size_t)(b.c2d[(size_t)(cb + x)] - cb)] = prim[(size_t)v * NTi + x];
+            for (int32_t g = 0; g < Gi; ++g) vg[(size_t)g] = prim[(size_t)v * NTi + Ni + g];
+            for (int k = 0; k < 2; ++k) {
+                upload_values(c, c->q[k], (size_t)v * c->NT + cb, vi);
+                upload_values(c, c->q[k], (size_t)v * c->NT + b.N_tot + gb, vg);
+            }
+        }
+        const int m1 = -1;
+        hip_check(hipMemcpy(b.conv + inst, &m1, sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+        b.have_state[(size_t)inst] = 1;
+        b.buf_of[(size_t)inst] = c->cur;
+        b.w_dirty = true;
+        c->have_state = true;
+        return PSST_OK;
+    } catch (const std::exception &e) {
+        bt->err = e.what();
+        return PSST_HIP_ERROR;
+    }
+}
+
+psst_status psst_batch_get_state(psst_batch *bt, int32_t inst, double *prim) {
+    if (!bt || !prim) return PSST_BAD_ARG;
+    psst_ctx *c = bt->ctx;
+    BatchState &b = *c->batch;
+    if (inst < 0 || inst >= b.n) {
+        bt->err = "instance index out of range";
+        return PSST_BAD_ARG;
+    }
+    try {
+        const int32_t cb = b.cell_begin[(size_t)inst], Ni = b.cell_begin[(size_t)inst + 1] - cb;
+        const int32_t gb = b.ghost_begin[(size_t)inst], Gi = b.ghost_begin[(size_t)inst + 1] - gb;
+        const size_t NTi = (size_t)Ni + Gi;
+        const void *src = c->q[b.buf_of[(size_t)inst]];
+        for (int v = 0; v < c->dim + 3; ++v) {
+            std::vector<double> vi = download_values(c, src, (size_t)v * c->NT + cb, (size_t)Ni);
+            std::vector<double> vg = download_values(c, src, (size_t)v * c->NT + b.N_tot + gb, (size_t)Gi);
+            for (int32_t x = 0; x < Ni; ++x) prim[(size_t)v * NTi + x] = vi[(size_t)(b.c2d[(size_t)(cb + x)] - cb)];
+            for (int32_t g = 0; g < Gi; ++g) prim[(size_t)v * NTi + Ni + g] = vg[(size_t)g];
+        }
+        return PSST_OK;
+    } catch (const std::exception &e) {
+        bt->err = e.what();
+        return PSST_HIP_ERROR;
+    }
+}
+
+psst_status psst_batch_set_residual_reference(psst_batch *bt, int32_t inst, const double *res0) {
+    if (!bt || !res0) return PSST_BAD_ARG;
+    psst_ctx *c = bt->ctx;
+    BatchState &b = *c->batch;
+    if (inst < 0 || inst >= b.n) {
+        bt->err = "instance index out of range";
+        return PSST_BAD_ARG;
+    }
+    if (hipMemcpy(b.res0 + (size_t)inst * b.ws, res0, sizeof(double) * b.ws, hipMemcpyHostToDevice) != hipSuccess) {
+        bt->err = "hipMemcpy failed";
+        return PSST_HIP_ERROR;
+    }
+    return PSST_OK;
+}
+
+psst_status psst_batch_set_graph(psst_batch *bt, int32_t enable) {
+    if (!bt) return PSST_BAD_ARG;
+    bt->ctx->batch->use_graph = enable != 0;
+    return PSST_OK;
+}
+
+psst_status psst_batch_run(psst_batch *bt, int32_t max_iter, double tol, int32_t *iters, int32_t *status,
+                           double *drop_out, double *res_hist) {
+    if (!bt) return PSST_BAD_ARG;
+    psst_ctx *c = bt->ctx;
+    BatchState &b = *c->batch;
+    for (int i = 0; i < b.n; ++i)
+        if (!b.have_state[(size_t)i]) {
+            bt->err = "psst_batch_run: instance " + std::to_string(i) + " has no state";
+            return PSST_BAD_ARG;
+        }
+    if (max_iter <= 0) return PSST_OK;
+    const int n = b.n, ws = b.ws;
+    try {
+        batch_gather_states(c);
+        if (b.w_dirty) {
+            if (c->dim == 2) c->s.precision == 64 ? run_set_w<2, double>(c) : run_set_w<2, float>(c);
+            else c->s.precision == 64 ? run_set_w<3, double>(c) : run_set_w<3, float>(c);
+            b.w_dirty = false;
+        }
+        std::vector<int> conv((size_t)n), active((size_t)n);
+        hip_check(hipMemcpy(conv.data(), b.conv, sizeof(int) * n, hipMemcpyDeviceToHost), "hipMemcpy");
+        for (int i = 0; i < n; ++i) {
+            active[(size_t)i] = conv[(size_t)i] != -2;
+            if (conv[(size_t)i] != -2) conv[(size_t)i] = -1;
+        }
+        hip_check(hipMemcpy(b.active, active.data(), sizeof(int) * n, hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMemcpy(b.conv, conv.data(), sizeof(int) * n, hipMemcpyHostToDevice), "hipMemcpy");
+        launch_check(c, tol, c->cur, 1);
+        c->bd = BatchDev{b.list, b.count, b.ctr, b.tile_inst, b.res};
+        const int64_t it0 = b.iter_host;
+        int done = 0;
+        bool any = true, first = true;
+        std::vector<double> ring;
+        while (done < max_iter && any) {
+            const int m = std::min(b.H, max_iter - done);
+            for (int k = 0; k < m; ++k) {
+                if (first || !b.use_graph || c->timing) batch_iteration(c, tol);
+                else batch_graph_iteration(c, tol);
+                first = false;
+            }
+            if (res_hist) {
+                ring.resize((size_t)b.H * n * ws);
+                hip_check(hipMemcpyAsync(ring.data(), b.hist, ring.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                         c->stream),
+                          "hipMemcpy");
+            }
+            hip_check(hipMemcpyAsync(active.data(), b.active, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream),
+                      "hipMemcpy");
+            hip_check(hipStreamSynchronize(c->stream), "psst_batch_run");
+            if (c->timing) flush_timing(c);
+            if (res_hist)
+                for (int k = 0; k < m; ++k) {
+                    const size_t row = (size_t)((b.iter_host + k) % b.H);
+                    std::memcpy(res_hist + (size_t)(done + k) * n * ws, ring.data() + row * n * ws,
+                                sizeof(double) * n * ws);
+                }
+            b.iter_host += m;
+            done += m;
+            any = false;
+            for (int i = 0; i < n; ++i) any = any || active[(size_t)i];
+        }
+        c->bd = BatchDev{};
+        std::vector<int> frozen((size_t)n);
+        std::vector<double> drop((size_t)n);
+        hip_check(hipMemcpy(conv.data(), b.conv, sizeof(int) * n, hipMemcpyDeviceToHost), "hipMemcpy");
+        hip_check(hipMemcpy(frozen.data(), b.frozen_buf, sizeof(int) * n, hipMemcpyDeviceToHost), "hipMemcpy");
+        hip_check(hipMemcpy(drop.data(), b.drop, sizeof(double) * n, hipMemcpyDeviceToHost), "hipMemcpy");
+        psst_status worst = PSST_OK;
+        for (int i = 0; i < n; ++i) {
+            const int cv = conv[(size_t)i];
+            b.buf_of[(size_t)i] = active[(size_t)i] ? c->cur : frozen[(size_t)i];
+            psst_status si = PSST_OK;
+            if (cv == -2) si = PSST_NAN_DETECTED;
+            else if (cv < 0 && tol > 0.0) si = PSST_NO_CONVERGENCE;
+            if (iters) iters[i] = cv >= 0 ? (int32_t)(cv - it0 + 1) : done;
+            if (status) status[i] = si;
+            if (drop_out) drop_out[i] = drop[(size_t)i];
+            if (si == PSST_NAN_DETECTED) worst = PSST_NAN_DETECTED;
+            else if (si == PSST_NO_CONVERGENCE && worst == PSST_OK) worst = PSST_NO_CONVERGENCE;
+        }
+        if (worst == PSST_NAN_DETECTED) bt->err = "non-finite residual in at least one instance";
+        else if (worst == PSST_NO_CONVERGENCE) bt->err = "at least one instance did not reach the tolerance";
+        return worst;
+    } catch (const std::exception &e) {
+        c->bd = BatchDev{};
+        bt->err = e.what();
+        return PSST_HIP_ERROR;
+    }
+}
+
+psst_status psst_batch_set_precision(psst_batch *bt, int32_t precision) {
+    if (!bt) return PSST_BAD_ARG;
+    psst_status st = psst_set_precision(bt->ctx, precision);
+    if (st != PSST_OK) bt->err = bt->ctx->err;
+    return st;
+}
+
+psst_status psst_batch_info(const psst_batch *bt, int64_t *info) {
+    if (!bt || !info) return PSST_BAD_ARG;
+    const psst_ctx *c = bt->ctx;
+    info[0] = c->batch->n;
+    info[1] = c->N;
+    info[2] = c->G;
+    info[3] = c->F;
+    info[4] = c->n_tiles;
+    info[5] = c->tile_cells;
+    info[6] = c->tile_threads;
+    info[7] = c->batch->iter_host;
+    return PSST_OK;
+}
+
+void psst_batch_destroy(psst_batch *bt) { delete bt; }
+
+psst_status psst_autotune(const psst_mesh_desc *m, const psst_settings *s, const double *prim, int32_t trial_iters,
+                          int32_t max_cand, int32_t *cand, double *ms_per_iter, int32_t *n_cand, int32_t *best) {
+    if (!m || !s || !prim || trial_iters <= 0 || !best) {
+        g_create_error = "psst_autotune: bad argument";
+        return PSST_BAD_ARG;
+    }
+    // tile shapes: cells per tile x threads per workgroup (the kernels are
+    // built for 256 and 512 threads)
+    static const int shapes[][2] = {{128, 256}, {192, 256}, {256, 256}, {256, 512}, {384, 256},
+                                    {384, 512}, {512, 256}, {512, 512}, {768, 512}, {1024, 512}};
+    double best_ms = 1e300;
+    int k_out = 0;
+    best[0] = best[1] = 0;
+    for (const auto &sh : shapes) {
+        psst_settings t = *s;
+        t.fused = 1;
+        t.tile_cells = sh[0];
+        t.tile_threads = sh[1];
+        psst_ctx *c = nullptr;
+        double ms = -1.0;
+        if (create_impl(m, &t, {}, &c) == PSST_OK) {
+            if (c->fused && psst_set_state(c, prim) == PSST_OK) {
+                int32_t done = 0;
+                if (psst_run(c, 2, 0.0, nullptr, &done) == PSST_OK) {
+                    const auto t0 = std::chrono::steady_clock::now();
+                    if (psst_run(c, trial_iters, 0.0, nullptr, &done) == PSST_OK)
+                        ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
+                             trial_iters;
+                }
+            }
+            psst_ctx_destroy(c);
+        }
+        if (cand && k_out < max_cand) {
+            cand[2 * k_out] = sh[0];
+            cand[2 * k_out + 1] = sh[1];
+        }
+        if (ms_per_iter && k_out < max_cand) ms_per_iter[k_out] = ms;
+        ++k_out;
+        if (ms > 0.0 && ms < best_ms) {
+            best_ms = ms;
+            best[0] = sh[0];
+            best[1] = sh[1];
+        }
+    }
+    if (n_cand) *n_cand = std::min(k_out, std::max(max_cand, 0));
+    if (best[0] == 0) {
+        g_create_error = "psst_autotune: no tile shape ran";
+        return PSST_UNSUPPORTED;
+    }
+    return PSST_OK;
 }
 
 }  // extern "C"

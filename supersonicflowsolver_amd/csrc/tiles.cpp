@@ -32,42 +32,59 @@ uint64_t spread3(uint64_t x) {  // 21 bits -> every third bit of 63
 
 }  // namespace
 
-std::vector<int32_t> morton_order(int dim, int32_t N, int64_t NT, const double *centre) {
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int k = 0; k < dim; ++k)
-        for (int32_t c = 0; c < N; ++c) {
-            double v = centre[(size_t)k * NT + c];
-            lo[k] = std::min(lo[k], v);
-            hi[k] = std::max(hi[k], v);
-        }
-    // one common scale for all axes keeps tiles square in physical space
-    double span = 0.0;
-    for (int k = 0; k < dim; ++k) span = std::max(span, hi[k] - lo[k]);
-    if (span <= 0.0) span = 1.0;
-    const double levels = (dim == 2) ? 4294967295.0 : 2097151.0;
-    std::vector<std::pair<uint64_t, int32_t>> key((size_t)N);
-    for (int32_t c = 0; c < N; ++c) {
-        uint64_t q[3] = {0, 0, 0};
-        for (int k = 0; k < dim; ++k) {
-            double t = (centre[(size_t)k * NT + c] - lo[k]) / span;
-            t = std::min(std::max(t, 0.0), 1.0);
-            q[k] = (uint64_t)(t * levels);
-        }
-        uint64_t code = (dim == 2) ? (spread2(q[0]) | (spread2(q[1]) << 1))
-                                   : (spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2));
-        key[(size_t)c] = {code, c};
-    }
-    std::sort(key.begin(), key.end());
+std::vector<int32_t> morton_order(int dim, int32_t N, int64_t NT, const double *centre,
+                                  const std::vector<int32_t> &seg) {
+    std::vector<int32_t> bounds = seg.empty() ? std::vector<int32_t>{0} : seg;
+    bounds.push_back(N);
     std::vector<int32_t> d2c((size_t)N);
-    for (int32_t d = 0; d < N; ++d) d2c[(size_t)d] = key[(size_t)d].second;
+    std::vector<std::pair<uint64_t, int32_t>> key;
+    for (size_t sidx = 0; sidx + 1 < bounds.size(); ++sidx) {
+        const int32_t b = bounds[sidx], e = bounds[sidx + 1];
+        if (b > e || b < 0 || e > N) throw std::runtime_error("segments must be ascending cell ranges");
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int k = 0; k < dim; ++k)
+            for (int32_t c = b; c < e; ++c) {
+                double v = centre[(size_t)k * NT + c];
+                lo[k] = std::min(lo[k], v);
+                hi[k] = std::max(hi[k], v);
+            }
+        // one common scale for all axes keeps tiles square in physical space
+        double span = 0.0;
+        for (int k = 0; k < dim; ++k) span = std::max(span, hi[k] - lo[k]);
+        if (span <= 0.0) span = 1.0;
+        const double levels = (dim == 2) ? 4294967295.0 : 2097151.0;
+        key.resize((size_t)(e - b));
+        for (int32_t c = b; c < e; ++c) {
+            uint64_t q[3] = {0, 0, 0};
+            for (int k = 0; k < dim; ++k) {
+                double t = (centre[(size_t)k * NT + c] - lo[k]) / span;
+                t = std::min(std::max(t, 0.0), 1.0);
+                q[k] = (uint64_t)(t * levels);
+            }
+            uint64_t code = (dim == 2) ? (spread2(q[0]) | (spread2(q[1]) << 1))
+                                       : (spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2));
+            key[(size_t)(c - b)] = {code, c};
+        }
+        std::sort(key.begin(), key.end());
+        for (int32_t d = b; d < e; ++d) d2c[(size_t)d] = key[(size_t)(d - b)].second;
+    }
     return d2c;
 }
 
 TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t *f0, const int32_t *f1,
-                    const int32_t *voff, const int32_t *vface, const int32_t *loff, const int32_t *lface) {
+                    const int32_t *voff, const int32_t *vface, const int32_t *loff, const int32_t *lface,
+                    const std::vector<int32_t> &seg) {
     TilePlan p;
     p.tile = tile;
-    p.n_tiles = (N + tile - 1) / tile;
+    // tile ranges: consecutive runs of `tile` cells, restarting at every segment
+    std::vector<int32_t> starts;
+    {
+        std::vector<int32_t> bounds = seg.empty() ? std::vector<int32_t>{0} : seg;
+        bounds.push_back(N);
+        for (size_t sidx = 0; sidx + 1 < bounds.size(); ++sidx)
+            for (int32_t c = bounds[sidx]; c < bounds[sidx + 1]; c += tile) starts.push_back(c);
+    }
+    p.n_tiles = (int32_t)starts.size();
     const int64_t NT = (int64_t)N + G;
     for (int32_t c = 0; c < N; ++c) {
         p.W_visit = std::max(p.W_visit, voff[c + 1] - voff[c]);
@@ -83,7 +100,7 @@ TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t
     std::vector<int32_t> lface_of((size_t)F, -1);   // global face -> local face in the current tile
     std::vector<int32_t> tile_faces, halo_cells;
     for (int32_t t = 0; t < p.n_tiles; ++t) {
-        const int32_t b = t * tile, e = std::min(N, b + tile);
+        const int32_t b = starts[(size_t)t], e = (t + 1 < p.n_tiles) ? starts[(size_t)t + 1] : N;
         p.cell_begin[(size_t)t] = b;
         const int32_t nc = e - b;
         for (int32_t c = b; c < e; ++c) slot_of[(size_t)c] = c - b;

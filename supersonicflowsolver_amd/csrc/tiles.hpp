@@ -18,7 +18,11 @@ namespace psst {
 
 // Device position of every interior cell (Morton order of the centroids,
 // ties by canonical index).  d2c[d] = canonical index of device cell d.
-std::vector<int32_t> morton_order(int dim, int32_t N, int64_t NT, const double *centre /* [dim][NT] */);
+// seg (optional) lists the first canonical cell of each independent
+// segment (instances of a batch, seg[0] = 0): cells are ordered within their
+// segment, and segments keep their order.
+std::vector<int32_t> morton_order(int dim, int32_t N, int64_t NT, const double *centre /* [dim][NT] */,
+                                  const std::vector<int32_t> &seg = {});
 
 struct TilePlan {
     int32_t tile = 256;                 // nominal cells per tile
@@ -41,7 +45,9 @@ struct TilePlan {
 
 // f0/f1: device cell ids per global face; voff/vface and loff/lface: per
 // device interior cell, global face ids in face-loop and face-list order.
+// seg (optional): first device cell of each segment; no tile spans two.
 TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t *f0, const int32_t *f1,
-                    const int32_t *voff, const int32_t *vface, const int32_t *loff, const int32_t *lface);
+                    const int32_t *voff, const int32_t *vface, const int32_t *loff, const int32_t *lface,
+                    const std::vector<int32_t> &seg = {});
 
 }  // namespace psst
