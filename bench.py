@@ -50,7 +50,82 @@ def parse():
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="profile mode: first copy a buffer of this many MiB (a kernel of known traffic that "
                          "calibrates the FETCH_SIZE/WRITE_SIZE counters)")
+    # batched sweep workload
+    ap.add_argument("--workload", default="wedge", choices=("wedge", "sweep"),
+                    help="wedge: the 4.9M-cell iteration throughput; sweep: batched solves of many wedges to a "
+                         "residual tolerance")
+    ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--sweep-seed", type=int, default=3)
+    ap.add_argument("--cells-min", type=int, default=20000)
+    ap.add_argument("--cells-max", type=int, default=400000)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--switch-tol", type=float, default=1e-3)
+    ap.add_argument("--max-iter", type=int, default=60000, help="sweep: iteration budget cap per instance "
+                    "(each instance gets min(this, 20 nx))")
+    ap.add_argument("--sweep-precision", type=int, default=64, choices=(32, 64),
+                    help="32: FP32 phase, then FP64 to the tolerance; 64: FP64 throughout")
+    ap.add_argument("--compare-fp64", type=int, default=0, help="sweep: also solve the batch in FP64 throughout")
+    ap.add_argument("--graph", type=int, default=1)
     return ap.parse_args()
+
+
+def run_sweep(a):
+    """Batched sweep: `instances` wedges (cells log-uniform, Mach and ramp
+    angle uniform, seeded), all solved to a `tol` residual drop in one batch
+    (persistent work-queue launches, hipGraph replay, autotuned tiles)."""
+    import numpy as np
+
+    import supersonicflowsolver_amd as ss
+    from supersonicflowsolver_amd.sweep import build_cases, draw_instances, solve_to_tolerance
+
+    def log(msg):
+        print("bench: " + msg, file=sys.stderr, flush=True)
+
+    inst = draw_instances(a.instances, a.sweep_seed, cells=(a.cells_min, a.cells_max))
+    tmp = tempfile.mkdtemp(prefix="psst_sweep_")
+    t0 = time.perf_counter()
+    cases = build_cases(inst, tmp, cfl=0.8, time_mode=2, rk_levels=4, flux=3, order=2, grad=1, limiter=2)
+    N = np.array([c.n_cells for c in cases], dtype=np.int64)
+    log("%d instances, %d cells (min %d, max %d), host setup %.1f s" %
+        (len(cases), N.sum(), N.min(), N.max(), time.perf_counter() - t0))
+    rep = int(np.argsort(N)[len(N) // 2])
+    t0 = time.perf_counter()
+    best, table = ss.autotune(cases[rep], trial_iters=10, precision=a.sweep_precision)
+    log("autotune on instance %d (%d cells): best tile %s, %.1f s; %s" %
+        (rep, N[rep], best, time.perf_counter() - t0, [(t[0], round(t[1], 4)) for t in table]))
+
+    def solve(precision):
+        t0 = time.perf_counter()
+        b = ss.Batch(cases, precision=precision, tile_cells=best[0], tile_threads=best[1])
+        b.set_graph(a.graph)
+        log("batch context (precision %d): %s, create %.1f s" % (precision, b.info(), time.perf_counter() - t0))
+        budget = [min(a.max_iter, 20 * d["nx"]) for d in inst]
+        r = solve_to_tolerance(b, tol=a.tol, switch_tol=a.switch_tol, budget=budget, log=log)
+        b.close()
+        return r
+
+    r = solve(a.sweep_precision)
+    it_tot = r["iters32"] + r["iters64"]
+    conv = int(np.sum(r["outcome"] == 0))
+    cell_its = float(np.sum(N * it_tot))
+    out = {"metric": "batched sweep: wall time until every instance reaches a %g residual drop" % a.tol,
+           "value": round(r["wall"], 3), "unit": "s", "n_gpus": 1, "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32+f64" if a.sweep_precision == 32 else "f64", "data": "synthetic",
+           "config": {"workload": "sweep of %d compression-ramp wedges" % len(cases), "instances": len(cases),
+                      "cells_total": int(N.sum()), "cells_range": [a.cells_min, a.cells_max],
+                      "mach_range": [1.5, 4.0], "ramp_deg_range": [5.0, 25.0], "seed": a.sweep_seed,
+                      "scheme": "HLLC, DM/MinMod, Green-Gauss, RK4, local time step 2, CFL 0.8",
+                      "tile": list(best), "graph": bool(a.graph)},
+           "converged": conv, "diverged": int(np.sum(r["outcome"] == 2)),
+           "budget_spent": int(np.sum(r["outcome"] == 1)), "cell_updates_per_s": round(cell_its / r["wall"], 1),
+           "wall_fp32_phase_s": round(r["wall32"], 3), "wall_fp64_phase_s": round(r["wall64"], 3),
+           "iters_fp32": {"median": int(np.median(r["iters32"])), "max": int(r["iters32"].max())},
+           "iters_fp64": {"median": int(np.median(r["iters64"])), "max": int(r["iters64"].max())}}
+    if a.compare_fp64 and a.sweep_precision == 32:
+        r64 = solve(64)
+        out["fp64_only"] = {"wall_s": round(r64["wall"], 3), "converged": int(np.sum(r64["outcome"] == 0)),
+                            "iters": {"median": int(np.median(r64["iters64"])), "max": int(r64["iters64"].max())}}
+    print(json.dumps(out), flush=True)
 
 
 def alg_bytes(kind, N, G, F, nnz, dim, b, last_stage=False):
@@ -94,6 +169,8 @@ def main():
         print("bench.py: this build runs on one GPU only (the path's multi-GPU sharding is a design note); "
               "--gpus %d is not supported" % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.workload == "sweep":
+        return run_sweep(a)
     import numpy as np
 
     import supersonicflowsolver_amd as ss

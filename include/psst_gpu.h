@@ -227,6 +227,9 @@ psst_status psst_batch_set_residual_reference(psst_batch *b, int32_t inst, const
 psst_status psst_batch_run(psst_batch *b, int32_t max_iter, double tol, int32_t *iters, int32_t *status,
                            double *drop, double *res_hist);
 psst_status psst_batch_set_precision(psst_batch *b, int32_t precision);
+/* Excludes (0) or includes (1, default) an instance in later psst_batch_run
+ * calls; an excluded instance keeps its state and reports 0 iterations. */
+psst_status psst_batch_set_enabled(psst_batch *b, int32_t inst, int32_t enabled);
 /* 1: replay iterations from a captured hipGraph (default); 0: launch each kernel. */
 psst_status psst_batch_set_graph(psst_batch *b, int32_t enable);
 /* info[0..7] = instances, union N, G, F, tiles, cells per tile, threads per tile, iterations run */

@@ -319,6 +319,7 @@ class Batch:
             raise PsstError(st, lib().psst_batch_last_error(None).decode())
         self._h = h
         self.n = n
+        self.precision = int(self._settings[0].precision)
         for i, c in enumerate(self.cases):
             self.set_state(i, c.initial_state())
 
@@ -361,6 +362,11 @@ class Batch:
 
     def set_precision(self, precision):
         self._check(lib().psst_batch_set_precision(self._h, int(precision)))
+        self.precision = int(precision)
+
+    def set_enabled(self, i, enabled):
+        """Excludes or includes instance i in later runs."""
+        self._check(lib().psst_batch_set_enabled(self._h, int(i), int(bool(enabled))))
 
     def set_graph(self, enable):
         self._check(lib().psst_batch_set_graph(self._h, int(bool(enable))))
@@ -371,11 +377,13 @@ class Batch:
         keys = ("instances", "n_cells", "n_ghosts", "n_faces", "tiles", "tile_cells", "tile_threads", "iterations")
         return dict(zip(keys, (int(x) for x in a)))
 
-    def run(self, max_iter, tol=0.0, history=False, allow_no_convergence=False):
+    def run(self, max_iter, tol=0.0, history=False, allow_no_convergence=False, allow_failures=False):
         """Runs every instance for up to max_iter iterations (each stops at its
         first iteration with max_j res_j/res0_j <= tol when tol > 0).  Returns a
         dict of per-instance ``iters``, ``status``, ``drop`` and, with
-        history=True, ``hist`` [max_iter, n, dim+2]."""
+        history=True, ``hist`` [max_iter, n, dim+2].  Raises on any instance
+        that did not converge (unless allow_no_convergence) or went
+        non-finite (unless allow_failures); the statuses say which."""
         n = self.n
         iters = np.zeros(n, dtype=np.int32)
         status = np.zeros(n, dtype=np.int32)
@@ -387,7 +395,7 @@ class Batch:
         out = {"iters": iters, "status": status, "drop": drop}
         if history:
             out["hist"] = hist[:max_iter]
-        if st == 4 and allow_no_convergence:
+        if (st == 4 and allow_no_convergence) or (st in (3, 4) and allow_failures):
             return out
         self._check(st)
         return out

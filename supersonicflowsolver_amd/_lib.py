@@ -101,6 +101,7 @@ _SIGS = {
                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "psst_batch_set_precision": (C.c_int, [C.c_void_p, C.c_int32]),
     "psst_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
+    "psst_batch_set_enabled": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "psst_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "psst_batch_context": (C.c_void_p, [C.c_void_p]),
     "psst_batch_destroy": (None, [C.c_void_p]),

@@ -1382,7 +1382,7 @@ struct BatchState {
     int32_t N_tot = 0;
     std::vector<int32_t> cell_begin, ghost_begin, tile_begin;  // [n+1] union ranges (device numbering)
     std::vector<int32_t> c2d;                                  // union canonical -> device
-    std::vector<char> have_state;
+    std::vector<char> have_state, enabled;
     std::vector<int32_t> buf_of;  // state buffer holding each instance's latest state
     int *tile_inst = nullptr, *tile_begin_dev = nullptr, *list = nullptr, *count = nullptr, *ctr = nullptr;
     unsigned long long *res = nullptr;
@@ -2843,6 +2843,7 @@ psst_status psst_batch_create(int32_t n, const psst_mesh_desc *meshes, const pss
         bs->c2d.resize((size_t)NT);
         for (int64_t d = 0; d < NT; ++d) bs->c2d[(size_t)c->d2c[(size_t)d]] = (int32_t)d;
         bs->have_state.assign((size_t)n, 0);
+        bs->enabled.assign((size_t)n, 1);
         bs->buf_of.assign((size_t)n, 0);
         const int ws = dim + 2;
         psst_ctx *cx = c;
@@ -2948,6 +2949,17 @@ psst_status psst_batch_set_residual_reference(psst_batch *bt, int32_t inst, cons
     return PSST_OK;
 }
 
+psst_status psst_batch_set_enabled(psst_batch *bt, int32_t inst, int32_t enabled) {
+    if (!bt) return PSST_BAD_ARG;
+    BatchState &b = *bt->ctx->batch;
+    if (inst < 0 || inst >= b.n) {
+        bt->err = "instance index out of range";
+        return PSST_BAD_ARG;
+    }
+    b.enabled[(size_t)inst] = enabled != 0;
+    return PSST_OK;
+}
+
 psst_status psst_batch_set_graph(psst_batch *bt, int32_t enable) {
     if (!bt) return PSST_BAD_ARG;
     bt->ctx->batch->use_graph = enable != 0;
@@ -2976,7 +2988,7 @@ psst_status psst_batch_run(psst_batch *bt, int32_t max_iter, double tol, int32_t
         std::vector<int> conv((size_t)n), active((size_t)n);
         hip_check(hipMemcpy(conv.data(), b.conv, sizeof(int) * n, hipMemcpyDeviceToHost), "hipMemcpy");
         for (int i = 0; i < n; ++i) {
-            active[(size_t)i] = conv[(size_t)i] != -2;
+            active[(size_t)i] = conv[(size_t)i] != -2 && b.enabled[(size_t)i];
             if (conv[(size_t)i] != -2) conv[(size_t)i] = -1;
         }
         hip_check(hipMemcpy(b.active, active.data(), sizeof(int) * n, hipMemcpyHostToDevice), "hipMemcpy");
@@ -3028,7 +3040,8 @@ psst_status psst_batch_run(psst_batch *bt, int32_t max_iter, double tol, int32_t
             psst_status si = PSST_OK;
             if (cv == -2) si = PSST_NAN_DETECTED;
             else if (cv < 0 && tol > 0.0) si = PSST_NO_CONVERGENCE;
-            if (iters) iters[i] = cv >= 0 ? (int32_t)(cv - it0 + 1) : done;
+            if (!b.enabled[(size_t)i] && cv != -2) si = PSST_OK;
+            if (iters) iters[i] = cv >= 0 ? (int32_t)(cv - it0 + 1) : (b.enabled[(size_t)i] ? done : 0);
             if (status) status[i] = si;
             if (drop_out) drop_out[i] = drop[(size_t)i];
             if (si == PSST_NAN_DETECTED) worst = PSST_NAN_DETECTED;

@@ -70,3 +70,80 @@ def make_case(inst, workdir, name, jitter=0.1, jitter_seed=1, numbering=1, **cas
     p = wedge_params(nx=inst["nx"], ny=inst["ny"], ramp_deg=inst["theta"], jitter=jitter, jitter_seed=jitter_seed,
                      numbering=numbering)
     return Case.wedge(p, prefix)
+
+
+def build_cases(instances, workdir, threads=16, prefix="s", **case_kw):
+    """Opens every instance as a Case, generating meshes on `threads` host
+    threads (the generator runs in C++ and releases the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(k):
+        return make_case(instances[k], workdir, "%s%d" % (prefix, k), **case_kw)
+
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        return list(ex.map(one, range(len(instances))))
+
+
+def solve_to_tolerance(batch, tol=1e-8, switch_tol=1e-3, chunk=1000, budget=None, diverge=1e3, stall=0.7, log=None):
+    """Runs a batch until every instance's residual drop reaches `tol`.
+
+    If the batch is in FP32, each instance first runs in FP32 until its drop
+    reaches `switch_tol` or stops improving (FP32 face-flux sums floor the
+    drop near 1e-4: an instance whose drop, below 1e-2, fell by less than a
+    factor `stall` over the last `chunk` iterations leaves the FP32 phase);
+    then the batch switches to FP64 and runs to `tol`.  An instance is given
+    up when its drop exceeds `diverge` (or turns non-finite) or when it has
+    used its iteration budget (`budget`, per instance; default unlimited).
+    Returns per-instance iteration counts of both phases, the final drop,
+    the outcome (0 converged, 1 budget spent, 2 diverged) and the wall time
+    of each phase (seconds)."""
+    import time
+
+    import numpy as np
+
+    n = batch.n
+    budget = np.full(n, 1 << 40, dtype=np.int64) if budget is None else np.asarray(budget, dtype=np.int64)
+    iters = {32: np.zeros(n, dtype=np.int64), 64: np.zeros(n, dtype=np.int64)}
+    outcome = np.full(n, -1, dtype=np.int64)
+    drop = np.full(n, np.inf)
+    walls = {32: 0.0, 64: 0.0}
+
+    def phase(prec, target):
+        t0 = time.perf_counter()
+        running = set(int(i) for i in np.nonzero(outcome < 0)[0])
+        for i in range(n):
+            batch.set_enabled(i, i in running)
+        prev = {}
+        total = 0
+        while running:
+            out = batch.run(chunk, tol=target, allow_failures=True)
+            total += chunk
+            iters[prec] += out["iters"]
+            for i in sorted(running):
+                d = float(out["drop"][i])
+                drop[i] = d
+                used = iters[32][i] + iters[64][i]
+                if out["status"][i] == 3 or not d < diverge:
+                    outcome[i] = 2
+                elif out["status"][i] == 0:
+                    if prec == 64 or target <= tol:
+                        outcome[i] = 0
+                elif used >= budget[i]:
+                    outcome[i] = 1
+                elif not (prec == 32 and i in prev and d < 1e-2 and d > stall * prev[i]):
+                    prev[i] = d
+                    continue
+                batch.set_enabled(i, False)
+                running.discard(i)
+            if log:
+                log("fp%d phase: %d iterations, %d instances still running" % (prec, total, len(running)))
+        walls[prec] = time.perf_counter() - t0
+
+    if batch.precision == 32:
+        phase(32, switch_tol)
+        batch.set_precision(64)
+    phase(64, tol)
+    for i in range(n):
+        batch.set_enabled(i, True)
+    return {"iters32": iters[32], "iters64": iters[64], "drop": drop, "outcome": outcome, "wall32": walls[32],
+            "wall64": walls[64], "wall": walls[32] + walls[64]}
