@@ -106,11 +106,25 @@ _SIGS = {
     "psst_batch_context": (C.c_void_p, [C.c_void_p]),
     "psst_batch_destroy": (None, [C.c_void_p]),
     "psst_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    # warm-start MLP (include/psst_mlp.h)
+    "psst_mlp_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "psst_mlp_param_count": (C.c_int64, [C.c_void_p]),
+    "psst_mlp_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "psst_mlp_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "psst_mlp_forward": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float)]),
+    "psst_mlp_loss_grad": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_float)]),
+    "psst_mlp_set_data": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64]),
+    "psst_mlp_train": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_uint64,
+                                 C.POINTER(C.c_float)]),
+    "psst_mlp_train_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "psst_mlp_destroy": (None, [C.c_void_p]),
+    "psst_mlp_last_error": (C.c_char_p, [C.c_void_p]),
 }
 
 
 def exported_symbols():
-    """Names of the C-ABI entry points the header declares."""
+    """Names of the C-ABI entry points the headers (psst_gpu.h, psst_mlp.h) declare."""
     return sorted(_SIGS)
 
 

@@ -1,7 +1,7 @@
 """Builds the package's native library ``libpsst.so`` in-tree with hipcc for
 gfx950: the host C++ (Fluent reader, grid flattening, settings, writers,
-generator) and the HIP kernels behind the C-ABI declared in
-``include/psst_gpu.h``."""
+generator), the HIP solver kernels behind the C-ABI declared in
+``include/psst_gpu.h`` and the warm-start MLP kernels of ``include/psst_mlp.h``."""
 import os
 import subprocess
 import sys
@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpsst.so")
 
 HOST_SOURCES = ["mesh_io.cpp", "grid.cpp", "setup.cpp", "meshgen.cpp", "meshgen3d.cpp", "host_api.cpp", "tiles.cpp"]
-DEVICE_SOURCES = ["psst_device.hip"]
+DEVICE_SOURCES = ["psst_device.hip", "mlp.hip"]
 
 # -ffp-contract=off: every multiply and add is rounded separately, as in the
 # x86-64 reference build (no FMA), which FP64 bitwise parity requires.

@@ -1,0 +1,79 @@
+"""Warm-start MLP kernels (BF16 MFMA forward/backward, Adam) against the CPU
+restatement in oracle/mlp_oracle.py: the GPU computes with FP32 accumulation
+what the oracle computes with FP64 accumulation at the same BF16 rounding
+points, so outputs and gradients agree to FP32-accumulation level (rare
+one-ulp BF16 differences after near-ties).  Needs an MI355X."""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+from supersonicflowsolver_amd.warmstart import MLP
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (4, 256, 5, 4)
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (n, 4)).astype(np.float32)
+    y = np.stack([np.sin(np.pi * x[:, 0]), np.cos(np.pi * x[:, 1]), x[:, 2] * x[:, 3], x[:, 0] ** 2 - 0.3],
+                 axis=1).astype(np.float32)
+    return x, y
+
+
+def _layer_slices(dims):
+    out, off = [], 0
+    for fi, fo in mo.layer_shapes(*dims):
+        out.append(slice(off, off + fi * fo))
+        off += fi * fo
+        out.append(slice(off, off + fo))
+        off += fo
+    return out
+
+
+@pytest.mark.parametrize("n", [64, 1000, 4097])
+def test_forward_and_gradients_match_oracle(n):
+    net = MLP(*DIMS, seed=7)
+    p = net.params()
+    assert p.shape == (265476,)
+    x, y = _data(n, 1)
+    # perturb the freshly initialised biases so every term is exercised
+    p[_layer_slices(DIMS)[1]] = 0.1
+    net.set_params(p)
+    out_gpu = net.forward(x)
+    out_cpu = mo.forward(p, x, DIMS)
+    err = np.abs(out_gpu - out_cpu).max() / max(1.0, np.abs(out_cpu).max())
+    print("n=%d forward max rel err %.2e" % (n, err), flush=True)
+    assert err < 2e-3
+    loss_gpu, g_gpu = net.loss_grad(x, y)
+    loss_cpu, g_cpu = mo.loss_grad(p, x, y, DIMS)
+    assert abs(loss_gpu - loss_cpu) / loss_cpu < 2e-3
+    for k, sl in enumerate(_layer_slices(DIMS)):
+        a, b = g_gpu[sl].astype(np.float64), g_cpu[sl]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print("n=%d param block %d: rel L2 %.2e" % (n, k, rel), flush=True)
+        assert rel < 1e-2, (k, rel)
+    _, g_exact = mo.grad_exact(p, x, y, DIMS)
+    assert np.linalg.norm(g_gpu - g_exact) / np.linalg.norm(g_exact) < 0.1
+    net.close()
+
+
+def test_training_decreases_loss_and_is_deterministic():
+    x, y = _data(60000, 2)
+    losses, params = [], []
+    for rep in range(2):
+        net = MLP(*DIMS, seed=3)
+        net.set_data(x, y)
+        h = net.train(300, batch=4096, lr=2e-3, lr_end=1e-4, seed=11)
+        losses.append(h)
+        params.append(net.params())
+        st = net.train_stats()
+        print("ms per step %.3f, %.1f TFLOP/s in the hidden GEMMs" %
+              (st["ms_per_step"], st["gemm_flop_per_step"] / st["ms_per_step"] / 1e9), flush=True)
+        net.close()
+    assert np.array_equal(params[0], params[1])
+    assert np.array_equal(losses[0], losses[1])
+    h = losses[0]
+    print("loss first %.4f last %.5f" % (h[:10].mean(), h[-10:].mean()), flush=True)
+    assert h[-10:].mean() < 0.1 * h[:10].mean()

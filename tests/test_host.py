@@ -1,5 +1,5 @@
 """Host side of the library (CPU only): the C-ABI library loads and exports
-every entry point of include/psst_gpu.h; the host reader, grid flattening
+every entry point of include/psst_gpu.h and include/psst_mlp.h; the host reader, grid flattening
 and settings files reproduce the oracle's setup; the generator's meshes
 round-trip through the Fluent writer and reader."""
 import ctypes
@@ -15,8 +15,10 @@ from oracle.oracle import Oracle
 
 
 def test_library_exports_every_declared_symbol():
-    with open(os.path.join(ROOT, "include", "psst_gpu.h")) as f:
-        text = f.read()
+    text = ""
+    for h in ("psst_gpu.h", "psst_mlp.h"):
+        with open(os.path.join(ROOT, "include", h)) as f:
+            text += f.read()
     declared = set(re.findall(r"\b(psst_[a-z_0-9]+)\s*\(", text))
     assert len(declared) >= 20
     L = ctypes.CDLL(ss._lib.LIB_PATH)
