@@ -51,7 +51,7 @@ def parse():
                     help="profile mode: first copy a buffer of this many MiB (a kernel of known traffic that "
                          "calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     # batched sweep workload
-    ap.add_argument("--workload", default="wedge", choices=("wedge", "sweep"),
+    ap.add_argument("--workload", default="wedge", choices=("wedge", "sweep", "warmstart"),
                     help="wedge: the 4.9M-cell iteration throughput; sweep: batched solves of many wedges to a "
                          "residual tolerance")
     ap.add_argument("--instances", type=int, default=512)
@@ -66,7 +66,72 @@ def parse():
                     help="32: FP32 phase, then FP64 to the tolerance; 64: FP64 throughout")
     ap.add_argument("--compare-fp64", type=int, default=0, help="sweep: also solve the batch in FP64 throughout")
     ap.add_argument("--graph", type=int, default=1)
+    # learned warm start workload
+    ap.add_argument("--train-instances", type=int, default=256)
+    ap.add_argument("--train-mesh", default="384x128", help="nx x ny of the training solves")
+    ap.add_argument("--eval-instances", type=int, default=128)
+    ap.add_argument("--eval-mesh", default="960x320", help="nx x ny of the held-out solves")
+    ap.add_argument("--train-steps", type=int, default=6000)
+    ap.add_argument("--train-batch", type=int, default=16384)
+    ap.add_argument("--budget-factor", type=int, default=12, help="iteration budget per solve: this x nx")
     return ap.parse_args()
+
+
+def run_warmstart(a):
+    """Learned warm start: training solves (seed 4) -> MLP training on the
+    GPU -> held-out solves (seed 5) from the freestream and from the MLP's
+    prediction, both to the same residual tolerance."""
+    import numpy as np
+
+    from supersonicflowsolver_amd.sweep import draw_instances
+    from supersonicflowsolver_amd.warmstart import MLP, ab_test, converged_fields
+
+    def log(msg):
+        print("bench: " + msg, file=sys.stderr, flush=True)
+
+    tnx, tny = (int(v) for v in a.train_mesh.split("x"))
+    enx, eny = (int(v) for v in a.eval_mesh.split("x"))
+    tmp = tempfile.mkdtemp(prefix="psst_warm_")
+    tr = draw_instances(a.train_instances, 4)
+    X, Y, rt, wall_data = converged_fields(tr, tmp, tnx, tny, tol=a.tol, budget_factor=a.budget_factor)
+    n_ok = int(np.sum(rt["outcome"] == 0))
+    log("training data: %d points from %d/%d converged %dx%d solves, %.1f s" % (len(X), n_ok, len(tr), tnx, tny,
+                                                                               wall_data))
+    net = MLP(4, 256, 5, 4, seed=1)
+    net.set_data(X, Y)
+    t0 = time.perf_counter()
+    hist = net.train(a.train_steps, batch=a.train_batch, lr=2e-3, lr_end=2e-5, seed=2)
+    t_train = time.perf_counter() - t0
+    st = net.train_stats()
+    log("MLP training: %d steps in %.1f s (%.3f ms/step, hidden GEMMs %.1f TFLOP/s), loss %.5f -> %.6f" %
+        (a.train_steps, t_train, st["ms_per_step"], st["gemm_flop_per_step"] / st["ms_per_step"] / 1e9,
+         float(hist[:50].mean()), float(hist[-200:].mean())))
+    ev = draw_instances(a.eval_instances, 5)
+    r = ab_test(net, ev, tmp, enx, eny, tol=a.tol, budget_factor=a.budget_factor)
+    pa, pb = r["plain"], r["warm"]
+    ok = (pa["outcome"] == 0) & (pb["outcome"] == 0)
+    cells = r["cells"]
+    out = {"metric": "time-to-%g residual: NN warm vs plain (batched, held-out Mach/angle)" % a.tol,
+           "value": round(pa["wall"] / pb["wall"], 4), "unit": "x (plain wall / warm wall)", "n_gpus": 1,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64 solver, bf16 MLP",
+           "data": "synthetic",
+           "config": {"workload": "learned warm start", "train": "%d solves at %dx%d (seed 4)" % (len(tr), tnx, tny),
+                      "eval": "%d solves at %dx%d (seed 5)" % (len(ev), enx, eny),
+                      "mlp": "4 -> 256 x 5 (SiLU) -> 4, BF16 MFMA, FP32 accumulation, Adam",
+                      "train_steps": a.train_steps, "train_batch": a.train_batch},
+           "plain_wall_s": round(pa["wall"], 3), "warm_wall_s": round(pb["wall"], 3),
+           "both_converged": int(ok.sum()),
+           "iters_plain": {"median": int(np.median(pa["iters64"][ok])), "sum": int(pa["iters64"][ok].sum())},
+           "iters_warm": {"median": int(np.median(pb["iters64"][ok])), "sum": int(pb["iters64"][ok].sum())},
+           "cell_iters_ratio": round(float(np.sum(cells[ok] * pa["iters64"][ok]) /
+                                           max(1, np.sum(cells[ok] * pb["iters64"][ok]))), 4),
+           "surrogate_rms_err": {"median": round(float(np.median(r["surrogate_rel_l2"][ok])), 5),
+                                 "max": round(float(np.max(r["surrogate_rel_l2"][ok])), 5)},
+           "predict_s": round(r["predict_s"], 3), "train_s": round(t_train, 2),
+           "train_ms_per_step": round(st["ms_per_step"], 4),
+           "train_gemm_tflops": round(st["gemm_flop_per_step"] / st["ms_per_step"] / 1e9, 2),
+           "train_data_s": round(wall_data, 2)}
+    print(json.dumps(out), flush=True)
 
 
 def run_sweep(a):
@@ -171,6 +236,8 @@ def main():
         sys.exit(2)
     if a.workload == "sweep":
         return run_sweep(a)
+    if a.workload == "warmstart":
+        return run_warmstart(a)
     import numpy as np
 
     import supersonicflowsolver_amd as ss

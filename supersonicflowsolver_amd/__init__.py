@@ -50,12 +50,14 @@ def write_wedge_msh(params, path):
 
 def write_case_files(prefix, *, cfl=0.8, time_mode=2, iterations=2000, save_step=5000, rk_levels=4,
                      rk_custom=None, flux=3, order=2, grad=1, limiter=2, mach=2.5, p=101325.0, T=300.0,
-                     molar_mass=0.029, cp=None, init_mode=0, bottom_bc="15", wall_names=("symm", "wedge", "wall", "up")):
+                     molar_mass=0.029, cp=None, init_mode=0, bottom_bc="15", wall_names=("symm", "wedge", "wall", "up"),
+                     dim=2, in_name="in", out_name="out"):
     """Writes the five settings files of a wedge case in the reference's formats
     (psstFiniteVolumeGrid.cpp:36-194, :403-717).  gamma = 1.4 is realised through
     cp = 3.5 R/M with the reference's R = 8.31 (quirk Q10); the inflow speed is
-    mach * sqrt(gamma rm T).  .bvp/.btq are written without a trailing newline
-    (the reference re-reads the last line otherwise, quirk Q11)."""
+    mach * sqrt(gamma rm T) along x (dim velocity components in 3D).
+    .bvp/.btq are written without a trailing newline (the reference re-reads
+    the last line otherwise, quirk Q11)."""
     R = 8.31
     rm = R / molar_mass
     if cp is None:
@@ -79,16 +81,19 @@ def write_case_files(prefix, *, cfl=0.8, time_mode=2, iterations=2000, save_step
         f.write("%d\t\t//order: 1 - firstOrder, 2 - secondOrder\n" % order)
         f.write("%d\t\t//grad: 1 - GreenGauss, 2 - LeastSquare\n" % grad)
         f.write("%d\t\t//limiters: 1 - VanAldaba, 2 - MinMod, 3 - VanLeer" % limiter)
-    lines = ["in\t\t23\t%r\t\t0.0\t%r\t//u,v,p" % (float(u), float(p))]
+    zeros = "\t0.0" * (dim - 1)
+    lines = ["%s\t\t23\t%r\t%s\t%r\t//u,v,p" % (in_name, float(u), zeros, float(p))]
     lines += ["%s\t\t%s" % (n, bottom_bc) for n in wall_names]
-    lines += ["out\t\t26"]
+    lines += ["%s\t\t26" % out_name]
     with open(prefix + ".bvp", "w") as f:
         f.write("\n".join(lines))
-    lines = ["in\t\t27\t%r\t//T" % float(T)] + ["%s\t\t18" % n for n in wall_names] + ["out\t\t25"]
+    lines = (["%s\t\t27\t%r\t//T" % (in_name, float(T))] + ["%s\t\t18" % n for n in wall_names] +
+             ["%s\t\t25" % out_name])
     with open(prefix + ".btq", "w") as f:
         f.write("\n".join(lines))
     with open(prefix + ".par", "w") as f:
-        f.write("%r 0.0  //velocity\n%r\t\t//Temperature\n%r\t//pressure\n" % (float(u), float(T), float(p)))
+        f.write("%r%s  //velocity\n%r\t\t//Temperature\n%r\t//pressure\n" % (float(u), " 0.0" * (dim - 1), float(T),
+                                                                                float(p)))
     with open(prefix + ".prop", "w") as f:
         f.write("%r\t\t//M\n%r       //cp" % (float(molar_mass), float(cp)))
     return {"u_inf": float(u), "gamma": float(gamma), "rm": float(rm), "cp": float(cp)}

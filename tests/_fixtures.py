@@ -11,6 +11,7 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 GENERATED_CASES = ["c1_lc1", "c1_lc2", "refmesh40x120_lc2"]
 MATRIX_CASES = sorted(d for d in (os.listdir(GOLDEN) if os.path.isdir(GOLDEN) else []) if d.startswith("m_"))
+THREE_D_CASES = sorted(d for d in (os.listdir(GOLDEN) if os.path.isdir(GOLDEN) else []) if d.startswith("r3d_"))
 
 
 def unpack(name, workdir):
@@ -34,6 +35,10 @@ def unpack(name, workdir):
     exp = np.load(os.path.join(src, "expected.npz"), allow_pickle=False)
     with open(os.path.join(src, "meta.json")) as f:
         meta = json.load(f)
+    if mesh is None and "mesh" in meta:  # shared mesh of several fixtures
+        mesh = os.path.join(workdir, meta["mesh"] + ".msh")
+        with lzma.open(os.path.join(GOLDEN, "_meshes", meta["mesh"] + ".msh.xz")) as f, open(mesh, "wb") as g:
+            g.write(f.read())
     return mesh, prefix, {"prim": exp["prim"], "residual": [str(x) for x in exp["residual"]]}, meta
 
 

@@ -14,6 +14,8 @@ Fixture kinds:
                      its residual file.
   m_*                a small matrix of schemes on a jittered 10x30 wedge and on
                      the reference's quad2D.msh, 12 iterations each.
+  r3d_*              3D: the reference's wedge3D.msh and quad3D.msh hex meshes
+                     (stored once under _meshes/), Green-Gauss schemes.
 
 Every expected output comes from running the reference's own sources
 (built by `make -C oracle ref`), never from this package's code.
@@ -97,10 +99,13 @@ def ref_wedge80x240():
     save_expected(d, vals, 2, res, {"iterations": 100, "source": "reference build, wedge.inp with maxIter=100"})
 
 
-def generated_case(name, params, case_kw, iterations, mesh_src=None):
+def generated_case(name, params, case_kw, iterations, mesh_src=None, shared_mesh=None):
+    """shared_mesh: store the mesh once under _meshes/<shared_mesh>.msh.xz
+    (several fixtures on the same mesh) instead of in the fixture dir."""
     import supersonicflowsolver_amd as ss
     d = os.path.join(HERE, name)
     os.makedirs(d, exist_ok=True)
+    dim = case_kw.get("dim", 2)
     with tempfile.TemporaryDirectory() as w:
         if mesh_src is None:
             mesh_path = os.path.join(w, "mesh.msh")
@@ -110,15 +115,23 @@ def generated_case(name, params, case_kw, iterations, mesh_src=None):
             with open(mesh_path, "wb") as f:
                 f.write(lf(mesh_src))
         with open(mesh_path, "rb") as f:
-            write_xz(os.path.join(d, "mesh.msh.xz"), f.read())
+            data = f.read()
+        if shared_mesh:
+            os.makedirs(os.path.join(HERE, "_meshes"), exist_ok=True)
+            write_xz(os.path.join(HERE, "_meshes", shared_mesh + ".msh.xz"), data)
+        else:
+            write_xz(os.path.join(d, "mesh.msh.xz"), data)
         kw = dict(case_kw)
         kw["iterations"] = iterations
         ss.write_case_files(os.path.join(w, "case"), **kw)
         for ext in ("inp", "bvp", "btq", "par", "prop"):
             shutil.copy(os.path.join(w, "case." + ext), d)
         vals, res = run_reference(w, "mesh.msh", "case")
-    save_expected(d, vals, 2, res, {"iterations": iterations, "wedge_params": params, "case": kw,
-                                    "mesh_source": mesh_src or "supersonicflowsolver_amd wedge generator"})
+    meta = {"iterations": iterations, "wedge_params": params, "case": kw,
+            "mesh_source": mesh_src or "supersonicflowsolver_amd wedge generator"}
+    if shared_mesh:
+        meta["mesh"] = shared_mesh
+    save_expected(d, vals, dim, res, meta)
 
 
 C1_MESH = dict(nx=120, ny=40, ramp_deg=15.0)
@@ -143,9 +156,35 @@ MATRIX = [
 ]
 
 
+# 3D: the reference's own hex meshes (wedge3D.msh: 4800 hexahedra with slip
+# front/back walls; quad3D.msh: 2x2x2 hexahedra).  Least squares is left out:
+# the reference's 3D least-squares gradients are zero (its alpha[3] write is
+# out of bounds, psstGradCalculation.cpp:336-339).
+W3D = dict(dim=3, wall_names=("up", "bottom", "frontAndBack"), mach=2.5, save_step=100000)
+Q3D = dict(dim=3, wall_names=("symm",), mach=2.5, save_step=100000)
+THREE_D = [
+    # name, mesh, case defaults, flux, order, limiter, time_mode, rk, iterations, cfl
+    ("r3d_wedge_hllc_o2_gg_mm_lc2_rk4", "wedge3D", W3D, 3, 2, 2, 2, 4, 40, 0.8),
+    ("r3d_wedge_hll_o2_gg_va_lc1_rk2", "wedge3D", W3D, 2, 2, 1, 1, 2, 20, 0.6),
+    ("r3d_wedge_ausm_o2_gg_vl_lc0_rk3", "wedge3D", W3D, 1, 2, 3, 0, 3, 20, 0.6),
+    ("r3d_quad_hll_o1_lc1_rk1", "quad3D", Q3D, 2, 1, 3, 1, 1, 12, 0.5),
+    ("r3d_quad_hllc_o2_gg_va_lc2_rk4", "quad3D", Q3D, 3, 2, 1, 2, 4, 12, 0.2),
+    ("r3d_quad_ausm_o2_gg_mm_lc1_rk2", "quad3D", Q3D, 1, 2, 2, 1, 2, 12, 0.5),
+    ("r3d_quad_hllc_o1_lc2_rk5", "quad3D", Q3D, 3, 1, 3, 2, 5, 12, 0.2),
+]
+
+
+def three_d():
+    for name, mesh, base, flux, order, lim, lc, rk, iters, cfl in THREE_D:
+        kw = dict(base, cfl=cfl, rk_levels=rk, flux=flux, order=order, grad=1, limiter=lim, time_mode=lc)
+        generated_case(name, None, kw, iters, mesh_src=os.path.join(REF, mesh + ".msh"), shared_mesh=mesh)
+
+
 def main():
     if not os.path.exists(REF_DUMP):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    if sys.argv[1:] == ["3d"]:
+        return three_d()
     ref_wedge80x240()
     for lc in (1, 2):
         generated_case("c1_lc%d" % lc, C1_MESH, dict(C1_CASE, time_mode=lc), 2000)
@@ -155,6 +194,7 @@ def main():
     # the reference's own 40x120 mesh (18.4 degree ramp) with the C1 scheme
     generated_case("refmesh40x120_lc2", None, dict(C1_CASE, time_mode=2), 200,
                    mesh_src=os.path.join(REF, "wedge40_120.msh"))
+    three_d()
 
 
 if __name__ == "__main__":
