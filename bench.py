@@ -93,7 +93,7 @@ def run_warmstart(a):
     enx, eny = (int(v) for v in a.eval_mesh.split("x"))
     tmp = tempfile.mkdtemp(prefix="psst_warm_")
     tr = draw_instances(a.train_instances, 4)
-    X, Y, rt, wall_data = converged_fields(tr, tmp, tnx, tny, tol=a.tol, budget_factor=a.budget_factor)
+    X, Y, rt, wall_data = converged_fields(tr, tmp, tnx, tny, tol=a.tol, budget_factor=a.budget_factor, log=log)
     n_ok = int(np.sum(rt["outcome"] == 0))
     log("training data: %d points from %d/%d converged %dx%d solves, %.1f s" % (len(X), n_ok, len(tr), tnx, tny,
                                                                                wall_data))
@@ -107,7 +107,7 @@ def run_warmstart(a):
         (a.train_steps, t_train, st["ms_per_step"], st["gemm_flop_per_step"] / st["ms_per_step"] / 1e9,
          float(hist[:50].mean()), float(hist[-200:].mean())))
     ev = draw_instances(a.eval_instances, 5)
-    r = ab_test(net, ev, tmp, enx, eny, tol=a.tol, budget_factor=a.budget_factor)
+    r = ab_test(net, ev, tmp, enx, eny, tol=a.tol, budget_factor=a.budget_factor, log=log)
     pa, pb = r["plain"], r["warm"]
     ok = (pa["outcome"] == 0) & (pb["outcome"] == 0)
     cells = r["cells"]

@@ -84,6 +84,11 @@ typedef struct psst_mesh_desc {
                                      (psstGradCalculation.cpp:191-260); NULL unless grad == 2 */
     const double *lsq_beta;       /* [N] 3D least-squares beta; NULL unless grad == 2 */
     double min_volume;            /* smallest interior cell volume */
+    const double *face_shift;     /* [3][F] translation that carries neighbour 1's centre into
+                                     neighbour 0's frame across a periodic face (zero elsewhere);
+                                     NULL: no periodic faces.  No reference counterpart (the
+                                     reader knows Fluent's periodic zone type 12 by name only,
+                                     psstAnsysGridReader.h:140-143). */
 } psst_mesh_desc;
 
 /* One boundary zone: dynamic + thermal condition (psstBoundary, psstBoundaryCondition.h). */

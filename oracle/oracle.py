@@ -44,7 +44,7 @@ def lib():
         L.oracle_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         L.oracle_write_sav.argtypes = [C.c_void_p, C.c_char_p, C.c_long]
         L.oracle_limiter.restype = C.c_double
-        L.oracle_limiter.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.oracle_limiter.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
         L.oracle_face_flux.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double)]
@@ -142,9 +142,10 @@ class Oracle:
         lib().oracle_write_sav(self._h, str(path).encode(), int(iter_value))
 
 
-def limiter(kind, num, den):
-    """psi(num, den) of limiter kind 1 (Van Albada), 2 (MinMod), 3 (Van Leer)."""
-    return lib().oracle_limiter(int(kind), float(num), float(den))
+def limiter(kind, num, den, eps2=0.0):
+    """psi(num, den) of limiter kind 1 (Van Albada), 2 (MinMod), 3 (Van Leer),
+    4 (Venkatakrishnan, with its eps^2)."""
+    return lib().oracle_limiter(int(kind), float(num), float(den), float(eps2))
 
 
 def face_flux(flux, n, q0, q1, gamma=1.4, rm=287.0):

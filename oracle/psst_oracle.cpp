@@ -273,6 +273,11 @@ struct Face {
     double n[3] = {0, 0, 0};
     double A = 0.0;
     double c[3] = {0, 0, 0};
+    /* periodic faces (zone type 12, no reference counterpart): translation
+     * carrying neighbour 1 into neighbour 0's frame */
+    double sh[3] = {0, 0, 0};
+    bool periodic = false;
+    double own_beta = 0.0; /* periodic faces: their own Green-Gauss weight */
 };
 
 struct Vol {
@@ -315,6 +320,7 @@ struct Oracle {
     std::map<long, ZoneStore> vol_zones;
     std::vector<long> file_zone_order;
     double min_vol = -1.0;
+    double venkat_k = 5.0, venkat_eps2 = 0.0;  // limiter 4 (no reference counterpart)
     size_t n_faces_total = 0, n_cells_total = 0;
 
     /* settings */
@@ -423,7 +429,16 @@ void volume_initialize(Oracle &o, int vi) {
         for (int i = 0; i < 3; ++i) v.x[i] = f.c[i];
         return;
     }
-    auto VX = [&](const Face &f, int j, int k) { return o.xyz[3 * (f.vtx[j] - 1) + k]; };
+    /* a periodic face as this volume sees it: translated by -sh for neighbour 1 */
+    auto OFF = [&](const Face &f, int k) { return (f.periodic && f.nb[1] == vi) ? -f.sh[k] : 0.0; };
+    auto VX = [&](const Face &f, int j, int k) {
+        const double x = o.xyz[3 * (f.vtx[j] - 1) + k], t = OFF(f, k);
+        return t != 0.0 ? x + t : x;
+    };
+    auto FCc = [&](const Face &f, int k) {
+        const double t = OFF(f, k);
+        return t != 0.0 ? f.c[k] + t : f.c[k];
+    };
     if (dim == 2) {
         double avg[2] = {0, 0}, avg3[2] = {0, 0};
         for (int j = 0; j < 2; ++j) {
@@ -448,7 +463,7 @@ void volume_initialize(Oracle &o, int vi) {
     } else {
         double avg[3] = {0, 0, 0}, avg4[3] = {0, 0, 0};
         for (int j = 0; j < 3; ++j) {
-            for (size_t i = 0; i < nf; ++i) avg[j] += o.faces[v.faces[i]].c[j];
+            for (size_t i = 0; i < nf; ++i) avg[j] += FCc(o.faces[v.faces[i]], j);
             avg[j] /= (double)nf;
             avg4[j] = avg[j] * 4;
         }
@@ -461,7 +476,7 @@ void volume_initialize(Oracle &o, int vi) {
                 for (int k = 0; k < 3; ++k) {
                     a[k] = VX(f, j, k) - avg[k];
                     b[k] = VX(f, next, k) - avg[k];
-                    c[k] = f.c[k] - avg[k];
+                    c[k] = FCc(f, k) - avg[k];
                 }
                 double tv = std::abs(a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) +
                                      a[2] * (b[0] * c[1] - b[1] * c[0]));
@@ -494,7 +509,7 @@ void build_grid(Oracle &o, const RawMesh &m) {
                 zs.items.push_back((int)o.vols.size());
                 o.vols.push_back(v);
             }
-        } else if (z.bc == 2 || z.bc == 31) {
+        } else if (z.bc == 2 || z.bc == 31 || z.bc == 12) {
             ZoneStore &fs = o.face_zones[z.id];
             fs.bc = z.bc; fs.name = z.name;
             for (long j = 0; j < n; ++j) {
@@ -547,10 +562,32 @@ void build_grid(Oracle &o, const RawMesh &m) {
             face_initialize(o, f);
         }
     }
+    /* periodic zones: periodic along z with period P = the z extent of the
+     * vertices; a face's vertices lie in neighbour 0's frame, neighbour 1 sees
+     * the face translated by +P (sh = -P) */
+    {
+        bool any = false;
+        for (const RawZone &z : m.zones) any = any || z.bc == 12;
+        if (any) {
+            if (o.dim != 3) throw std::runtime_error("periodic zones need a 3D mesh");
+            double zmin = 1e300, zmax = -1e300;
+            for (size_t v = 0; v < o.xyz.size() / 3; ++v) {
+                zmin = std::min(zmin, o.xyz[3 * v + 2]);
+                zmax = std::max(zmax, o.xyz[3 * v + 2]);
+            }
+            for (const RawZone &z : m.zones) {
+                if (z.bc != 12) continue;
+                for (int fi : o.face_zones.at(z.id).items) {
+                    o.faces[fi].periodic = true;
+                    o.faces[fi].sh[2] = zmin - zmax;
+                }
+            }
+        }
+    }
     /* volume initialisation in file zone order, V_min over bc==1 (:885-905) */
     double mn = -1.0;
     for (const RawZone &z : m.zones) {
-        if (z.bc == 2 || z.bc == 31 || z.bc == 32) continue;
+        if (z.bc == 2 || z.bc == 31 || z.bc == 32 || z.bc == 12) continue;
         for (int vi : o.vol_zones.at(z.id).items) {
             volume_initialize(o, vi);
             if (z.bc == 1) {
@@ -652,7 +689,7 @@ void load_boundary(Oracle &o, const std::string &prefix) {
         if (!(in >> name >> bc_id)) break;
         long zid;
         ZoneStore &zs = find_zone_by_name(o, name, zid);
-        if (zs.bc == 1 || zs.bc == 2 || zs.bc == 31 || zs.bc == 32)
+        if (zs.bc == 1 || zs.bc == 2 || zs.bc == 12 || zs.bc == 31 || zs.bc == 32)
             throw std::runtime_error("Could not attach boundary condition to inner zone");
         BCDyn d;
         d.kind = (int)bc_id;
@@ -693,7 +730,7 @@ void load_boundary(Oracle &o, const std::string &prefix) {
         if (!(tq >> name >> bc_id)) break;
         long zid;
         ZoneStore &zs = find_zone_by_name(o, name, zid);
-        if (zs.bc == 1 || zs.bc == 2 || zs.bc == 31 || zs.bc == 32)
+        if (zs.bc == 1 || zs.bc == 2 || zs.bc == 12 || zs.bc == 31 || zs.bc == 32)
             throw std::runtime_error("Could not attach boundary condition to inner zone");
         BCThermal t;
         t.kind = (int)bc_id;
@@ -767,10 +804,16 @@ void thermal_calculate_at(Oracle &o, const BCThermal &bc, Vol &g) {
 /* Limiters (psstScalarLimiters.cpp) and Roe average (psstConvectiveFlow.cpp).*/
 /* ------------------------------------------------------------------------- */
 
-double limiter_value(int kind, double num, double den) {
+double limiter_value(int kind, double num, double den, double eps2 = 0.0) {
     const double eps = 1.0e-10;
     if ((num > 0.0 && den < 0.0) || (num < 0.0 && den > 0.0)) return 0.0;
     switch (kind) {
+    case 4: { /* Venkatakrishnan, ratio form (no reference counterpart):
+                 (a^2 + 2ab + eps2) / (a^2 + ab + 2b^2 + eps2) */
+        if (std::abs(den) < eps) return 1.0;
+        const double nn = num * num, nd = num * den, dd = den * den;
+        return (nn + 2.0 * nd + eps2) / (nn + nd + 2.0 * dd + eps2);
+    }
     case 1: { /* VanAlbada :17-31 */
         if (std::abs(den) < eps) return 1.0;
         double r = num / den;
@@ -822,7 +865,9 @@ void gg_initialize(Oracle &o) {
                 if (!v.boundary) {
                     double dist = 0.0;
                     for (int k = 0; k < o.dim; ++k) {
-                        double add = (f.c[k] - v.x[k]);
+                        double xc = v.x[k];
+                        if (j == 1 && f.periodic) xc = xc + f.sh[k];
+                        double add = (f.c[k] - xc);
                         add = add * add;
                         dist += add;
                     }
@@ -830,6 +875,7 @@ void gg_initialize(Oracle &o) {
                 }
             }
             o.gg_beta[i] = d[1] / (d[0] + d[1]);
+            if (f.periodic) o.faces[items[i]].own_beta = d[1] / (d[0] + d[1]);
         }
     }
 }
@@ -859,7 +905,7 @@ void gg_calculate(Oracle &o) {
             Vol *v[2] = {&o.vols[f.nb[0]], &o.vols[f.nb[1]]};
             double geom[2];
             for (int j = 0; j < 2; ++j) geom[j] = f.A / v[j]->V;
-            double beta = o.gg_beta[i];
+            double beta = f.periodic ? f.own_beta : o.gg_beta[i];
             double rbeta = 1 - beta;
             for (int n = 0; n < dim + 3; ++n) {
                 double tv = beta * *var_ptr(o, *v[0], n) + rbeta * *var_ptr(o, *v[1], n);
@@ -887,9 +933,12 @@ void lsq_initialize(Oracle &o) {
             double cf[6] = {0, 0, 0, 0, 0, 0};
             for (size_t j = 0; j < v.faces.size(); ++j) {
                 const Vol &nb = o.vols[v.nbs[j]];
+                const Face &fj = o.faces[v.faces[j]];
                 double diff[3], w = 0.0;
                 for (int k = 0; k < dim; ++k) {
-                    diff[k] = nb.x[k] - v.x[k];
+                    double xn = nb.x[k];
+                    if (fj.periodic && fj.sh[k] != 0.0) xn = (fj.nb[0] == items[i]) ? xn + fj.sh[k] : xn - fj.sh[k];
+                    diff[k] = xn - v.x[k];
                     w += diff[k] * diff[k];
                 }
                 double ws = 1.0 / w;
@@ -931,9 +980,12 @@ void lsq_calculate(Oracle &o) {
             const auto &r = o.lsq_r[i];
             for (size_t j = 0; j < v.faces.size(); ++j) {
                 Vol &nb = o.vols[v.nbs[j]];
+                const Face &fj = o.faces[v.faces[j]];
                 double diff[3], w = 0.0, alpha[3] = {0, 0, 0}, theta[3] = {0, 0, 0}, phi[6];
                 for (int k = 0; k < dim; ++k) {
-                    diff[k] = nb.x[k] - v.x[k];
+                    double xn = nb.x[k];
+                    if (fj.periodic && fj.sh[k] != 0.0) xn = (fj.nb[0] == items[i]) ? xn + fj.sh[k] : xn - fj.sh[k];
+                    diff[k] = xn - v.x[k];
                     w += diff[k] * diff[k];
                 }
                 for (int n = 0; n < dim; ++n) phi[n] = (nb.u[n] - v.u[n]);
@@ -982,7 +1034,7 @@ void reconstruct(Oracle &o, const Face &f, RVar rv[2]) {
     if (o.order == 1) return; /* psstFirstOrderReconstructor :28-48 */
     /* psstReconstructorDM::reconstruct_vals :70-134 */
     double RLR[3];
-    for (int j = 0; j < dim; ++j) RLR[j] = v[0]->x[j] - v[1]->x[j];
+    for (int j = 0; j < dim; ++j) RLR[j] = v[0]->x[j] - (f.periodic ? v[1]->x[j] + f.sh[j] : v[1]->x[j]);
     for (int i = 0; i < dim + 2; ++i) {
         double *p0 = (i < dim) ? &rv[0].u[i] : (i == dim ? &rv[0].p : &rv[0].T);
         double *p1 = (i < dim) ? &rv[1].u[i] : (i == dim ? &rv[1].p : &rv[1].T);
@@ -1001,8 +1053,8 @@ void reconstruct(Oracle &o, const Face &f, RVar rv[2]) {
         double dnl = *p1 - vlm;
         double nr = vrm - *p0;
         double dnr = *p0 - vrp;
-        *p0 += 0.5 * limiter_value(o.limiter, nr, dnr) * dnr;
-        *p1 += 0.5 * limiter_value(o.limiter, nl, dnl) * dnl;
+        *p0 += 0.5 * limiter_value(o.limiter, nr, dnr, o.venkat_eps2) * dnr;
+        *p1 += 0.5 * limiter_value(o.limiter, nl, dnl, o.venkat_eps2) * dnl;
     }
     for (int j = 0; j < 2; ++j) rv[j].rho = rv[j].p / (o.rm * rv[j].T);
 }
@@ -1167,7 +1219,7 @@ void convective_calculate(Oracle &o) {
     /* psstConvective*::calculate face loop: map order, skip bc 31 */
     for (auto &kv : o.face_zones) {
         if (kv.second.bc == 31) continue;
-        if (kv.second.bc != 2) {
+        if (kv.second.bc != 2 && kv.second.bc != 12) {
             for (int fi : kv.second.items) boundary_flux(o, o.faces[fi]);
         } else {
             for (int fi : kv.second.items) interior_flux(o, o.faces[fi]);
@@ -1268,7 +1320,20 @@ void solver_parameters(Oracle &o, const std::string &prefix) {
     if (o.grad == 1) gg_initialize(o);
     else lsq_initialize(o);
     o.limiter = (int)get_parameter(in);
-    if (o.limiter < 1 || o.limiter > 3) o.limiter = 0;
+    if (o.limiter < 1 || o.limiter > 4) o.limiter = 0;
+    {
+        // Venkatakrishnan's eps^2 = (K h)^3 with h the mean interior cell size
+        double vsum = 0.0;
+        long n = 0;
+        for (auto &kv : o.vol_zones)
+            if (kv.second.bc == 1)
+                for (int vi : kv.second.items) {
+                    vsum += o.vols[vi].V;
+                    ++n;
+                }
+        const double kh = o.venkat_k * std::pow(vsum / (double)n, 1.0 / o.dim);
+        o.venkat_eps2 = kh * kh * kh;
+    }
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1577,7 +1642,7 @@ int oracle_residual(void *h, double *R) {
 }
 
 /* Scalar limiter psi(num, den) (psstScalarLimiters.cpp). */
-double oracle_limiter(int kind, double num, double den) { return limiter_value(kind, num, den); }
+double oracle_limiter(int kind, double num, double den, double eps2) { return limiter_value(kind, num, den, eps2); }
 
 /* Interior-face flux of two states q = (u..., p, T, rho) for unit normal n
  * pointing toward state 0; returns 0, or 1 when the reference would throw. */

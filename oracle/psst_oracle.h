@@ -22,7 +22,7 @@ int oracle_get_w(void *h, double *w);
 int oracle_gradients(void *h, double *grads);
 int oracle_residual(void *h, double *R);
 int oracle_write_sav(void *h, const char *path, long iter_value);
-double oracle_limiter(int kind, double num, double den);
+double oracle_limiter(int kind, double num, double den, double eps2);
 int oracle_face_flux(int flux, int dim, double gamma, double rm, double cp, double cv, const double *n,
                      const double *q0, const double *q1, double *F);
 #ifdef __cplusplus

@@ -25,7 +25,7 @@ class MeshDesc(C.Structure):
         ("list_offsets", C.POINTER(C.c_int32)), ("list_faces", C.POINTER(C.c_int32)),
         ("ghost_face", C.POINTER(C.c_int32)), ("ghost_bc", C.POINTER(C.c_int32)),
         ("lsq_r", C.POINTER(C.c_double)), ("lsq_beta", C.POINTER(C.c_double)),
-        ("min_volume", C.c_double),
+        ("min_volume", C.c_double), ("face_shift", C.POINTER(C.c_double)),
     ]
 
 

@@ -126,7 +126,7 @@ Grid build_grid(const RawMesh &m) {
     }
     for (GridZone &z : g.zones) {
         const RawZone &rz = *by_id[z.id];
-        if (z.bc != 1 && z.bc != 2) {
+        if (z.bc != 1 && z.bc != 2 && z.bc != 12) {
             z.has_cells = true;
             z.cell_begin = n_int + n_gh;
             z.cell_count = (int32_t)(rz.last - rz.first + 1);
@@ -182,7 +182,7 @@ Grid build_grid(const RawMesh &m) {
                 int64_t cell = m.face_c[2 * fi + k];
                 int32_t ci;
                 if (cell == 0) {
-                    if (z.bc == 2) throw std::runtime_error("mesh: interior face without two cells");
+                    if (z.bc == 2 || z.bc == 12) throw std::runtime_error("mesh: interior face without two cells");
                     ci = z.cell_begin + j;
                     g.ghost_face[(size_t)(ci - n_int)] = cf;
                     g.ghost_zone[(size_t)(ci - n_int)] = zi;
@@ -192,7 +192,7 @@ Grid build_grid(const RawMesh &m) {
                 }
                 (k == 0 ? g.f0 : g.f1)[(size_t)cf] = ci;
             }
-            if (z.bc != 2) {
+            if (z.bc != 2 && z.bc != 12) {
                 bool g0 = g.f0[(size_t)cf] >= n_int, g1 = g.f1[(size_t)cf] >= n_int;
                 if (g0 == g1) throw std::runtime_error("mesh: boundary face must have exactly one ghost");
             }
@@ -252,16 +252,50 @@ Grid build_grid(const RawMesh &m) {
         }
         g.fa[cf] = fg.A;
     }
+    // ---- periodic faces (zone type 12, no reference counterpart): the mesh
+    //      is periodic along z with period P = its z extent; a periodic
+    //      face's vertices lie in cell(0)'s frame and cell(1) sees the face
+    //      translated by +P.  fshift = -P carries cell(1) into cell(0)'s frame. ----
+    g.fshift.assign(3 * (size_t)nf, 0.0);
+    g.periodic = false;
+    {
+        double zmin = 1e300, zmax = -1e300;
+        for (const GridZone &z : g.zones) {
+            if (z.bc != 12 || !z.has_faces) continue;
+            if (dim != 3) throw std::runtime_error("mesh: periodic zones need a 3D mesh");
+            if (!g.periodic) {
+                for (int64_t v = 0; v < m.n_vertices(); ++v) {
+                    zmin = std::min(zmin, m.xyz[3 * (size_t)v + 2]);
+                    zmax = std::max(zmax, m.xyz[3 * (size_t)v + 2]);
+                }
+                g.periodic = true;
+            }
+            for (int32_t j = 0; j < z.face_count; ++j) g.fshift[2 * (size_t)nf + (size_t)(z.face_begin + j)] = zmin - zmax;
+        }
+    }
     // ---- cell geometry (psstVolume::initialize, face-list order) ----
     g.vol.assign((size_t)NT, 0.0);
     g.cx.assign(3 * (size_t)NT, 0.0);
     std::vector<size_t> file_face_of((size_t)nf);
     for (int64_t fi = 0; fi < m.n_faces(); ++fi) file_face_of[(size_t)canon_face[(size_t)fi]] = (size_t)fi;
-    auto VX = [&](int32_t cf, int j, int k) {
+    // vertex j of face cf as cell c sees it (a periodic face translated by
+    // +P for its cell(1)); FC likewise for the face centre
+    auto off = [&](int32_t cf, int64_t c, int k) {
+        return (g.periodic && g.f1[(size_t)cf] == c) ? -g.fshift[(size_t)k * nf + cf] : 0.0;
+    };
+    auto VXc = [&](int32_t cf, int64_t c, int j, int k) {
         size_t fi = file_face_of[(size_t)cf];
-        return m.xyz[3 * (size_t)(m.face_v[4 * fi + j] - 1) + k];
+        const double x = m.xyz[3 * (size_t)(m.face_v[4 * fi + j] - 1) + k];
+        const double o = off(cf, c, k);
+        return o != 0.0 ? x + o : x;
+    };
+    auto FC = [&](int32_t cf, int64_t c, int k) {
+        const double x = g.fc[(size_t)k * nf + cf];
+        const double o = off(cf, c, k);
+        return o != 0.0 ? x + o : x;
     };
     for (int64_t c = 0; c < NT; ++c) {
+        auto VX = [&](int32_t cf, int j, int k) { return VXc(cf, c, j, k); };
         int32_t b = g.list_off[(size_t)c], e = g.list_off[(size_t)c + 1];
         if (c >= n_int) {
             int32_t cf = g.ghost_face[(size_t)(c - n_int)];
@@ -295,7 +329,7 @@ Grid build_grid(const RawMesh &m) {
         } else {
             double avg[3] = {0, 0, 0}, avg4[3];
             for (int j = 0; j < 3; ++j) {
-                for (int32_t i = b; i < e; ++i) avg[j] += g.fc[(size_t)j * nf + g.list_face[(size_t)i]];
+                for (int32_t i = b; i < e; ++i) avg[j] += FC(g.list_face[(size_t)i], c, j);
                 avg[j] /= (double)nfc;
                 avg4[j] = avg[j] * 4;
             }
@@ -308,7 +342,7 @@ Grid build_grid(const RawMesh &m) {
                     for (int k = 0; k < 3; ++k) {
                         a[k] = VX(cf, j, k) - avg[k];
                         bb[k] = VX(cf, nx, k) - avg[k];
-                        cc[k] = g.fc[(size_t)k * nf + cf] - avg[k];
+                        cc[k] = FC(cf, c, k) - avg[k];
                     }
                     double tv = std::abs(a[0] * (bb[1] * cc[2] - bb[2] * cc[1]) - a[1] * (bb[0] * cc[2] - bb[2] * cc[0]) +
                                          a[2] * (bb[0] * cc[1] - bb[1] * cc[0]));
@@ -329,12 +363,14 @@ Grid build_grid(const RawMesh &m) {
     g.min_vol = mn;
     // ---- Green-Gauss beta, indexed by zone-local face index (quirk Q1,
     //      psstGradCalculation.cpp:30-70): the last face zone in ascending id
-    //      order that has a face with local index i defines beta[i]. ----
+    //      order that has a face with local index i defines beta[i].  Periodic
+    //      faces (no reference counterpart) keep their own beta. ----
     {
         int32_t max_local = 0;
         for (const GridZone &z : g.zones)
             if (z.has_faces) max_local = std::max(max_local, z.face_count);
         std::vector<double> beta_local((size_t)max_local, 0.0);
+        std::vector<std::pair<int32_t, double>> own;  // periodic faces keep their own beta
         for (const GridZone &z : g.zones) {
             if (!z.has_faces) continue;
             for (int32_t i = 0; i < z.face_count; ++i) {
@@ -346,7 +382,9 @@ Grid build_grid(const RawMesh &m) {
                     if (nb[j] < n_int) {
                         double dist = 0.0;
                         for (int k = 0; k < dim; ++k) {
-                            double add = (g.fc[(size_t)k * nf + cf] - g.cx[(size_t)k * NT + nb[j]]);
+                            double xc = g.cx[(size_t)k * NT + nb[j]];
+                            if (j == 1 && g.periodic) xc = xc + g.fshift[(size_t)k * nf + cf];
+                            double add = (g.fc[(size_t)k * nf + cf] - xc);
                             add = add * add;
                             dist += add;
                         }
@@ -354,10 +392,12 @@ Grid build_grid(const RawMesh &m) {
                     }
                 }
                 beta_local[(size_t)i] = d[1] / (d[0] + d[1]);
+                if (z.bc == 12) own.push_back({cf, beta_local[(size_t)i]});
             }
         }
         g.beta.assign((size_t)nf, 0.0);
         for (int32_t f = 0; f < nf; ++f) g.beta[(size_t)f] = beta_local[(size_t)face_local[(size_t)f]];
+        for (auto &fb : own) g.beta[(size_t)fb.first] = fb.second;
     }
     // ---- least-squares factors, indexed by zone-local cell index (quirk Q20,
     //      psstGradCalculation.cpp:191-260) ----
@@ -375,10 +415,16 @@ Grid build_grid(const RawMesh &m) {
                 double cf[6] = {0, 0, 0, 0, 0, 0};
                 for (int32_t l = g.list_off[(size_t)c]; l < g.list_off[(size_t)c + 1]; ++l) {
                     int32_t f = g.list_face[(size_t)l];
-                    int32_t nb = g.f0[(size_t)f] == c ? g.f1[(size_t)f] : g.f0[(size_t)f];
+                    const bool c_is_0 = g.f0[(size_t)f] == c;
+                    int32_t nb = c_is_0 ? g.f1[(size_t)f] : g.f0[(size_t)f];
                     double diff[3] = {0, 0, 0}, w = 0.0;
                     for (int k = 0; k < dim; ++k) {
-                        diff[k] = g.cx[(size_t)k * NT + nb] - g.cx[(size_t)k * NT + c];
+                        double xn = g.cx[(size_t)k * NT + nb];
+                        if (g.periodic) {
+                            const double sh = g.fshift[(size_t)k * nf + f];
+                            if (sh != 0.0) xn = c_is_0 ? xn + sh : xn - sh;
+                        }
+                        diff[k] = xn - g.cx[(size_t)k * NT + c];
                         w += diff[k] * diff[k];
                     }
                     double ws = 1.0 / w;
@@ -438,6 +484,7 @@ psst_mesh_desc make_mesh_desc(const Grid &g) {
     d.lsq_r = g.lsq_r.data();
     d.lsq_beta = g.lsq_beta.data();
     d.min_volume = g.min_vol;
+    d.face_shift = g.periodic ? g.fshift.data() : nullptr;
     return d;
 }
 

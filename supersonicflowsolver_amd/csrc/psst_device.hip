@@ -65,6 +65,7 @@ struct Geom {
     const int *__restrict__ gnb;      // [G]
     const int *__restrict__ gbc;      // [G]
     const int *__restrict__ rank;     // [NT] position in the reference's update loop (ghosts: >= N)
+    const double *__restrict__ rlr;   // [3][F] x_0 - x_1 per face (neighbour 1 in neighbour 0's frame)
 };
 
 struct BCDev {
@@ -73,7 +74,7 @@ struct BCDev {
 };
 
 struct Phys {
-    double gamma, rm, cp, cv, cfl, dt_uniform, venkat_k;
+    double gamma, rm, cp, cv, cfl, dt_uniform, venkat_eps2;
 };
 
 // Gradients are stored cell-major ("AoS"): the (dim+2) x dim components of a
@@ -164,11 +165,19 @@ __device__ __forceinline__ float fdiv<float>(float a, float b) {
 }
 
 // psstScalarLimiters.cpp:17-93; eps is absolute (physical units).
+// Venkatakrishnan (limiter 4, no reference counterpart): the smooth ratio
+// form psi = (a^2 + 2ab + eps2) / (a^2 + ab + 2b^2 + eps2), a = num, b = den,
+// which without eps2 is (r^2 + 2r) / (r^2 + r + 2) in r = a/b; eps2 =
+// (K h)^3 with h the mean cell size, as in Venkatakrishnan's cell limiter.
 template <typename T, int LIM>
-__device__ __forceinline__ T limiter(T num, T den) {
+__device__ __forceinline__ T limiter(T num, T den, T eps2) {
     const T eps = T(1.0e-10);
     if ((num > T(0) && den < T(0)) || (num < T(0) && den > T(0))) return T(0);
-    if (LIM == PSST_LIMITER_VAN_ALBADA) {
+    if (LIM == PSST_LIMITER_VENKATAKRISHNAN) {
+        if (fabs(den) < eps) return T(1);
+        const T nn = num * num, nd = num * den, dd = den * den;
+        return fdiv(nn + T(2) * nd + eps2, nn + nd + T(2) * dd + eps2);
+    } else if (LIM == PSST_LIMITER_VAN_ALBADA) {
         if (fabs(den) < eps) return T(1);
         T r = fdiv(num, den);
         return fdiv(r * r + r, r * r + T(1));
@@ -279,7 +288,8 @@ struct Side {
 // G0/G1 return the gradient component (var i, axis k) of neighbour 0 / 1;
 // RLR = x_0 - x_1.
 template <int DIM, typename T, int LIM, typename G0, typename G1>
-__device__ __forceinline__ void reconstruct_dm(Side<DIM, T> &r0, Side<DIM, T> &r1, const T *RLR, T rm, G0 g0, G1 g1) {
+__device__ __forceinline__ void reconstruct_dm(Side<DIM, T> &r0, Side<DIM, T> &r1, const T *RLR, T rm, T eps2, G0 g0,
+                                               G1 g1) {
     constexpr int NV = DIM + 2;
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
@@ -300,8 +310,8 @@ __device__ __forceinline__ void reconstruct_dm(Side<DIM, T> &r0, Side<DIM, T> &r
         T dnl = v1 - vlm;
         T nr = vrm - v0;
         T dnr = v0 - vrp;
-        *p0 = v0 + T(0.5) * limiter<T, LIM>(nr, dnr) * dnr;
-        *p1 = v1 + T(0.5) * limiter<T, LIM>(nl, dnl) * dnl;
+        *p0 = v0 + T(0.5) * limiter<T, LIM>(nr, dnr, eps2) * dnr;
+        *p1 = v1 + T(0.5) * limiter<T, LIM>(nl, dnl, eps2) * dnl;
     }
     r0.rho = fdiv(r0.p, rm * r0.T_);
     r1.rho = fdiv(r1.p, rm * r1.T_);
@@ -493,11 +503,11 @@ __global__ void __launch_bounds__(kBlock) k_flux(Geom g, State<T> s, Phys ph) {
         if (ORDER == 2) {
             T RLR[DIM];
 #pragma unroll
-            for (int k = 0; k < DIM; ++k) RLR[k] = T(g.cx[(size_t)k * g.NT + a0] - g.cx[(size_t)k * g.NT + a1]);
+            for (int k = 0; k < DIM; ++k) RLR[k] = T(g.rlr[(size_t)k * g.F + f]);
             const T *gr = s.grad;
             const size_t N = (size_t)g.N;
             reconstruct_dm<DIM, T, LIM>(
-                r[0], r[1], RLR, T(ph.rm),
+                r[0], r[1], RLR, T(ph.rm), T(ph.venkat_eps2),
                 [&](int i, int k) { return gr[(size_t)a0 * GradLayout<DIM>::STRIDE + i * DIM + k]; },
                 [&](int i, int k) { return gr[(size_t)a1 * GradLayout<DIM>::STRIDE + i * DIM + k]; });
         }
@@ -1036,7 +1046,7 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, con
                 load_grads<DIM, T>(s.grad, scell[l0], g0);
                 load_grads<DIM, T>(s.grad, scell[l1], g1);
                 reconstruct_dm<DIM, T, LIM>(
-                    r[0], r[1], RLR, T(ph.rm), [&](int i, int k) { return g0[i * DIM + k]; },
+                    r[0], r[1], RLR, T(ph.rm), T(ph.venkat_eps2), [&](int i, int k) { return g0[i * DIM + k]; },
                     [&](int i, int k) { return g1[i * DIM + k]; });
             }
             riemann_flux<DIM, T, FLUX>(r, nrm, ph, flow);
@@ -1660,6 +1670,8 @@ void run_stage_lim(psst_ctx *c, int k, unsigned long long *res_row) {
         return run_stage_lc<DIM, T, FLUX, 2, PSST_LIMITER_VAN_ALBADA>(c, k, res_row);
     case PSST_LIMITER_MINMOD:
         return run_stage_lc<DIM, T, FLUX, 2, PSST_LIMITER_MINMOD>(c, k, res_row);
+    case PSST_LIMITER_VENKATAKRISHNAN:
+        return run_stage_lc<DIM, T, FLUX, 2, PSST_LIMITER_VENKATAKRISHNAN>(c, k, res_row);
     default:
         return run_stage_lc<DIM, T, FLUX, 2, PSST_LIMITER_VAN_LEER>(c, k, res_row);
     }
@@ -1695,6 +1707,9 @@ void run_flux_lim(psst_ctx *c) {
         break;
     case PSST_LIMITER_MINMOD:
         launch(c, "flux", k_flux<DIM, T, FLUX, 2, PSST_LIMITER_MINMOD>, gr, bl, c->geom, st, c->ph);
+        break;
+    case PSST_LIMITER_VENKATAKRISHNAN:
+        launch(c, "flux", k_flux<DIM, T, FLUX, 2, PSST_LIMITER_VENKATAKRISHNAN>, gr, bl, c->geom, st, c->ph);
         break;
     default:
         launch(c, "flux", k_flux<DIM, T, FLUX, 2, PSST_LIMITER_VAN_LEER>, gr, bl, c->geom, st, c->ph);
@@ -1855,8 +1870,10 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
         if (s->rk_levels < 1 || s->rk_levels > 8) throw std::invalid_argument("rk_levels must be 1..8");
         if (s->flux < 1 || s->flux > 3) throw std::invalid_argument("flux must be 1 (AUSM), 2 (HLL) or 3 (HLLC)");
         if (s->order != 1 && s->order != 2) throw std::invalid_argument("order must be 1 or 2");
-        if (s->order == 2 && (s->limiter < 1 || s->limiter > 3))
-            throw std::invalid_argument("limiter must be 1..3 for second order");
+        if (s->order == 2 && (s->limiter < 1 || s->limiter > 4))
+            throw std::invalid_argument("limiter must be 1..4 for second order");
+        if (s->order == 2 && s->limiter == PSST_LIMITER_VENKATAKRISHNAN && !(s->venkat_k > 0.0))
+            throw std::invalid_argument("venkat_k must be positive");
         if (s->time_mode < 0 || s->time_mode > 2) throw std::invalid_argument("time_mode must be 0, 1 or 2");
         if (s->precision != 64 && s->precision != 32) throw std::invalid_argument("precision must be 64 or 32");
         if (s->order == 2 && s->grad == PSST_GRAD_LEAST_SQUARES && (!m->lsq_r || !m->lsq_beta))
@@ -1870,7 +1887,12 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
         c->s = *s;
         c->bc_host.assign(s->bc, s->bc + s->n_bc);
         c->s.bc = nullptr;
-        c->ph = Phys{s->gamma, s->rm, s->cp, s->cv, s->cfl, s->uniform_dt, s->venkat_k};
+        // Venkatakrishnan's eps^2 = (K h)^3, h = mean interior cell size
+        // (sum of volumes in canonical order, then the dim-th root)
+        double vsum = 0.0;
+        for (int i = 0; i < m->n_cells; ++i) vsum += m->cell_volume[i];
+        const double hbar = std::pow(vsum / m->n_cells, 1.0 / m->dim), kh = s->venkat_k * hbar;
+        c->ph = Phys{s->gamma, s->rm, s->cp, s->cv, s->cfl, s->uniform_dt, kh * kh * kh};
         const int dim = c->dim, N = c->N, G = c->G, F = c->F, NT = c->NT;
         // ---- validation of the canonical topology (every index in range) ----
         const int32_t *f0 = m->face_cells, *f1 = m->face_cells + F;
@@ -1944,6 +1966,15 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
             int f = m->ghost_face[i];
             gnb[(size_t)i] = (f0d[(size_t)f] == N + i) ? f1d[(size_t)f] : f0d[(size_t)f];
         }
+        // neighbour 1 of face f in neighbour 0's frame (periodic faces)
+        auto sh = [&](int k, int f) { return m->face_shift ? m->face_shift[(size_t)k * F + f] : 0.0; };
+        auto x1 = [&](int k, int f) {
+            const double x = m->cell_centre[(size_t)k * NT + f1[f]], t = sh(k, f);
+            return t != 0.0 ? x + t : x;
+        };
+        std::vector<double> rlr3(3 * (size_t)F, 0.0);
+        for (int f = 0; f < F; ++f)
+            for (int k = 0; k < dim; ++k) rlr3[(size_t)k * F + f] = m->cell_centre[(size_t)k * NT + f0[f]] - x1(k, f);
         std::vector<double> vol_d((size_t)NT), cx_d(3 * (size_t)NT, 0.0), fn3(3 * (size_t)F, 0.0);
         for (int d = 0; d < NT; ++d) {
             vol_d[(size_t)d] = m->cell_volume[d2c[(size_t)d]];
@@ -1971,9 +2002,13 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                              rg = m->lsq_r[(size_t)5 * N + i], beta = m->lsq_beta[i];
                 for (int k = loff[(size_t)d]; k < loff[(size_t)d + 1]; ++k) {
                     const int nb = lnb_c[(size_t)lsrc[(size_t)k]];
+                    const int fl = lface[(size_t)k];
                     double diff[3] = {0, 0, 0}, w = 0.0, alpha[3] = {0, 0, 0}, theta[3] = {0, 0, 0};
                     for (int x = 0; x < dim; ++x) {
-                        diff[x] = m->cell_centre[(size_t)x * NT + nb] - m->cell_centre[(size_t)x * NT + i];
+                        double xn = m->cell_centre[(size_t)x * NT + nb];
+                        const double t = sh(x, fl);
+                        if (t != 0.0) xn = (f0[fl] == i) ? xn + t : xn - t;
+                        diff[x] = xn - m->cell_centre[(size_t)x * NT + i];
                         w += diff[x] * diff[x];
                     }
                     double ws = 1.0 / w;
@@ -2060,6 +2095,7 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
         g.lface = dev_upload(c.get(), lface.data(), lface.size());
         g.lnb = dev_upload(c.get(), lnb.data(), lnb.size());
         g.lsqc = dev_upload(c.get(), lsqc.data(), lsqc.size());
+        g.rlr = dev_upload(c.get(), rlr3.data(), rlr3.size());
         g.S = dev_upload(c.get(), S.data(), S.size());
         g.gface = dev_upload(c.get(), m->ghost_face, (size_t)G);
         g.gnb = dev_upload(c.get(), gnb.data(), (size_t)G);
@@ -2102,8 +2138,7 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                     const int f = tp.face_id[(size_t)j];
                     for (int k = 0; k < dim; ++k) {
                         tfn[(size_t)k * TF + j] = m->face_normal[(size_t)k * F + f];
-                        trlr[(size_t)k * TF + j] =
-                            m->cell_centre[(size_t)k * NT + f0[f]] - m->cell_centre[(size_t)k * NT + f1[f]];
+                        trlr[(size_t)k * TF + j] = rlr3[(size_t)k * F + f];
                     }
                     tfa[(size_t)j] = m->face_area[f];
                     tbeta[(size_t)j] = m->face_beta[f];
@@ -2715,6 +2750,8 @@ psst_status psst_batch_create(int32_t n, const psst_mesh_desc *meshes, const pss
             throw std::invalid_argument("batches need an order-free time step (time_mode 0 or 2, or dt_ordering 1)");
         int64_t N = 0, G = 0, F = 0, nv = 0, nl = 0, nbc = 0;
         bool lsq = s0.order == 2 && s0.grad == PSST_GRAD_LEAST_SQUARES;
+        bool any_shift = false;
+        for (int i = 0; i < n; ++i) any_shift = any_shift || meshes[i].face_shift;
         for (int i = 0; i < n; ++i) {
             const psst_mesh_desc &m = meshes[i];
             if (m.dim != dim) throw std::invalid_argument("instances of a batch must share the dimension");
@@ -2732,7 +2769,8 @@ psst_status psst_batch_create(int32_t n, const psst_mesh_desc *meshes, const pss
         if (NT >= INT32_MAX / 2 || F >= INT32_MAX / 2 || nv >= INT32_MAX / 2 || nl >= INT32_MAX / 2)
             throw std::invalid_argument("batch too large for 32-bit indices");
         std::vector<double> vol((size_t)NT), cen((size_t)dim * NT), nrm((size_t)dim * F), area((size_t)F),
-            beta((size_t)F), lsq_r(lsq ? 6 * (size_t)N : 0), lsq_b(lsq ? (size_t)N : 0);
+            beta((size_t)F), lsq_r(lsq ? 6 * (size_t)N : 0), lsq_b(lsq ? (size_t)N : 0),
+            fshift(any_shift ? 3 * (size_t)F : 0, 0.0);
         std::vector<int32_t> fc(2 * (size_t)F), voff((size_t)N + 1), vf((size_t)nv), loff((size_t)N + 1), lf((size_t)nl),
             gface((size_t)G), gbc((size_t)G), seg((size_t)n);
         std::vector<psst_bc> bcs;
@@ -2759,6 +2797,8 @@ psst_status psst_batch_create(int32_t n, const psst_mesh_desc *meshes, const pss
             }
             for (int64_t f = 0; f < Fi; ++f) {
                 for (int k = 0; k < dim; ++k) nrm[(size_t)k * F + fb + f] = m.face_normal[(size_t)k * Fi + f];
+                if (m.face_shift)
+                    for (int k = 0; k < 3; ++k) fshift[(size_t)k * F + fb + f] = m.face_shift[(size_t)k * Fi + f];
                 area[(size_t)(fb + f)] = m.face_area[f];
                 beta[(size_t)(fb + f)] = m.face_beta[f];
                 const int32_t a = m.face_cells[f], b = m.face_cells[Fi + f];
@@ -2816,6 +2856,7 @@ psst_status psst_batch_create(int32_t n, const psst_mesh_desc *meshes, const pss
         um.lsq_r = lsq ? lsq_r.data() : nullptr;
         um.lsq_beta = lsq ? lsq_b.data() : nullptr;
         um.min_volume = vmin;
+        um.face_shift = any_shift ? fshift.data() : nullptr;
         psst_settings us = s0;
         us.fused = 1;
         us.bc = bcs.data();

@@ -106,6 +106,8 @@ struct Grid {
     std::vector<GridZone> zones;     // ascending id
     double min_vol = -1.0;
     std::vector<int64_t> original_cell;   // [N] 1-based file cell index
+    std::vector<double> fshift;      // [3][F] translation of cell 1 into cell 0's frame (periodic faces)
+    bool periodic = false;           // the mesh has periodic (type 12) face zones
 };
 
 Grid build_grid(const RawMesh &mesh);

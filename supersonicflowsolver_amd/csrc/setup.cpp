@@ -177,7 +177,7 @@ void load_settings(Case &cs, const std::string &prefix) {
         while (in >> name >> id) {
             int zi = zone_by_name(g, name);
             const GridZone &z = g.zones[(size_t)zi];
-            if (z.bc == 1 || z.bc == 2 || z.bc == 31 || z.bc == 32)
+            if (z.bc == 1 || z.bc == 2 || z.bc == 12 || z.bc == 31 || z.bc == 32)
                 throw std::runtime_error("Error: Could not attach boundary condition to inner zone!");
             psst_bc &b = cs.bc[(size_t)zi];
             b.dynamic_kind = (int32_t)id;
@@ -220,7 +220,7 @@ void load_settings(Case &cs, const std::string &prefix) {
         while (in >> name >> id) {
             int zi = zone_by_name(g, name);
             const GridZone &z = g.zones[(size_t)zi];
-            if (z.bc == 1 || z.bc == 2 || z.bc == 31 || z.bc == 32)
+            if (z.bc == 1 || z.bc == 2 || z.bc == 12 || z.bc == 31 || z.bc == 32)
                 throw std::runtime_error("Error: Could not attach boundary condition to inner zone!");
             psst_bc &b = cs.bc[(size_t)zi];
             b.thermal_kind = (int32_t)id;
@@ -246,7 +246,7 @@ void load_settings(Case &cs, const std::string &prefix) {
     }
     for (size_t zi = 0; zi < g.zones.size(); ++zi) {
         const GridZone &z = g.zones[zi];
-        if (z.bc == 1 || z.bc == 2) continue;
+        if (z.bc == 1 || z.bc == 2 || z.bc == 12) continue;
         if (!has_dyn[zi] || !has_th[zi])
             throw std::runtime_error("Error: boundary zone '" + z.name + "' needs both a .bvp and a .btq entry");
     }

@@ -198,6 +198,8 @@ def ab_test(mlp, instances, workdir, nx, ny, tol=1e-8, budget_factor=20, tile=No
     for i in range(n):
         b.set_state(i, fresh[i])
         b.set_residual_reference(i, res0[i])
+    if log:
+        log("plain (freestream) starts of %d instances" % n)
     ra = solve_to_tolerance(b, tol=tol, budget=budget, log=log)
     conv_a = [b.state(i) for i in range(n)]
     t0 = time.perf_counter()
@@ -206,6 +208,8 @@ def ab_test(mlp, instances, workdir, nx, ny, tol=1e-8, budget_factor=20, tile=No
     for i in range(n):
         b.set_state(i, warm[i])
         b.set_residual_reference(i, res0[i])
+    if log:
+        log("warm starts (network prediction, %.2f s)" % t_pred)
     rb = solve_to_tolerance(b, tol=tol, budget=budget, log=log)
     b.close()
     # surrogate error against the converged field: RMS over the interior

@@ -180,6 +180,22 @@ def test_limiter_closed_forms():
     assert [limiter(k, 0.5, 1e-12) for k in (1, 2, 3)] == [1.0, 1.0, 2.0]
 
 
+def test_venkatakrishnan_closed_forms():
+    """Limiter 4 (no reference counterpart): without eps^2 it is
+    Venkatakrishnan's psi(r) = (r^2 + 2r)/(r^2 + r + 2) (3/4 at r = 1, -> 1 as
+    r -> inf); zero for opposite signs; eps^2 pulls it towards 1 where both
+    differences are small; homogeneous of degree 0 in (num, den, sqrt(eps^2))."""
+    from oracle.oracle import limiter
+    for r in [0.1, 0.5, 1.0, 2.0, 7.0, 1e6]:
+        assert limiter(4, r, 1.0) == pytest.approx((r * r + 2 * r) / (r * r + r + 2), rel=1e-15)
+    assert limiter(4, 1.0, 1.0) == 0.75
+    assert limiter(4, 1e8, 1.0) == pytest.approx(1.0, abs=1e-7)
+    assert limiter(4, -1.0, 2.0) == 0.0 and limiter(4, 1.0, -2.0) == 0.0
+    assert limiter(4, 0.5, 1e-12) == 1.0
+    assert limiter(4, 1e-3, 2e-3, eps2=1.0) == pytest.approx(1.0, abs=1e-5)
+    assert limiter(4, 3.0, 6.0, eps2=9 * 0.25) == pytest.approx(limiter(4, 1.0, 2.0, eps2=0.25), rel=1e-15)
+
+
 def _physical_flux(n, q, gamma=1.4, rm=287.0):
     d = len(n)
     u = np.array(q[:d])
