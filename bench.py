@@ -34,6 +34,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx3", type=int, default=768)
+    ap.add_argument("--ny3", type=int, default=256)
+    ap.add_argument("--nz3", type=int, default=64)
+    ap.add_argument("--tets", type=int, default=0, help="wedge3d: six tetrahedra per hexahedron")
     ap.add_argument("--nx", type=int, default=3840)
     ap.add_argument("--ny", type=int, default=1280)
     ap.add_argument("--flux", default="hllc", choices=sorted(FLUX_IDS))
@@ -51,7 +55,7 @@ def parse():
                     help="profile mode: first copy a buffer of this many MiB (a kernel of known traffic that "
                          "calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     # batched sweep workload
-    ap.add_argument("--workload", default="wedge", choices=("wedge", "sweep", "warmstart"),
+    ap.add_argument("--workload", default="wedge", choices=("wedge", "wedge3d", "sweep", "warmstart"),
                     help="wedge: the 4.9M-cell iteration throughput; sweep: batched solves of many wedges to a "
                          "residual tolerance")
     ap.add_argument("--instances", type=int, default=512)
@@ -245,16 +249,26 @@ def main():
     numbering = {"column": 0, "rcm": 1, "random": 2}[a.numbering]
     tmp = tempfile.mkdtemp(prefix="psst_bench_")
     prefix = os.path.join(tmp, "case")
-    ss.write_case_files(prefix, cfl=0.8, time_mode=a.time_mode, iterations=a.steps, rk_levels=4,
-                        flux=FLUX_IDS[a.flux], order=2, grad=1, limiter=2, mach=2.5, p=101325.0, T=300.0)
-    params = ss.wedge_params(nx=a.nx, ny=a.ny, ramp_deg=15.0, jitter=0.1, jitter_seed=1, numbering=numbering,
-                             perm_seed=2)
+    three_d = a.workload == "wedge3d"
+    dim = 3 if three_d else 2
+    if three_d:
+        # 20 degree ramp extruded along z with periodic side faces; M 3, AUSM, Venkatakrishnan, least squares
+        ss.write_case_files(prefix, dim=3, wall_names=("symm", "wedge", "wall", "up"), cfl=0.8,
+                            time_mode=a.time_mode, iterations=a.steps, rk_levels=4, flux=FLUX_IDS["ausm"], order=2,
+                            grad=2, limiter=4, mach=3.0, p=101325.0, T=300.0)
+        params = ss.wedge_params(nx=a.nx3, ny=a.ny3, nz=a.nz3, depth=1.0, ramp_deg=20.0, jitter=0.0,
+                                 numbering=numbering, tets=a.tets, periodic_z=1)
+    else:
+        ss.write_case_files(prefix, cfl=0.8, time_mode=a.time_mode, iterations=a.steps, rk_levels=4,
+                            flux=FLUX_IDS[a.flux], order=2, grad=1, limiter=2, mach=2.5, p=101325.0, T=300.0)
+        params = ss.wedge_params(nx=a.nx, ny=a.ny, ramp_deg=15.0, jitter=0.1, jitter_seed=1, numbering=numbering,
+                                 perm_seed=2)
     t0 = time.perf_counter()
     case = ss.Case.wedge(params, prefix)
     t_setup = time.perf_counter() - t0
     N, G, F = case.n_cells, case.n_ghosts, case.n_faces
     nnz = int(case.mesh().visit_offsets[N])
-    print("bench: mesh %dx%d, %d cells, %d faces, host setup %.1f s" % (a.nx, a.ny, N, F, t_setup),
+    print("bench: %s mesh, %d cells, %d faces, host setup %.1f s" % ("3D" if three_d else "2D", N, F, t_setup),
           file=sys.stderr, flush=True)
 
     if a.profile and a.calib_copy_mb > 0:
@@ -307,7 +321,7 @@ def main():
     dom = max(per_iter, key=per_iter.get)
     kind = {"flux": "flux", "update": "update", "grad_gg": "grad", "grad_lsq": "grad", "bc": "bc",
             "stage": "stage", "stage_last": "stage"}.get(dom, dom)
-    nbytes = alg_bytes(kind, N, G, F, nnz, 2, b, last_stage=(dom == "stage_last"))
+    nbytes = alg_bytes(kind, N, G, F, nnz, dim, b, last_stage=(dom == "stage_last"))
     mean_ms = stats[dom][1]
     achieved = nbytes / (mean_ms * 1e-3) / 1e9
     # HBM traffic of the dominant kernel: rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs (each scaled by
@@ -323,15 +337,19 @@ def main():
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "alg_bytes_per_launch": int(nbytes), "mean_ms": round(mean_ms, 4)}
-    whole = survey_bytes_per_cell_iteration(2, b, N, F)
+    whole = survey_bytes_per_cell_iteration(dim, b, N, F)
     out = {
         "metric": METRIC, "value": round(cell_its, 1), "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(el / a.steps * 1e3, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if a.precision == 64 else "f32",
         "data": "synthetic (generated wedge mesh, freestream initial state)",
-        "config": {"workload": "2D wedge %dx%d quads (%d cells), jitter 10%% seed 1, %s numbering; %s + DM/MinMod + "
-                               "Green-Gauss, RK4, time step mode %d, CFL 0.8" % (a.nx, a.ny, N, a.numbering,
-                                                                              a.flux.upper(), a.time_mode),
+        "config": {"workload": ("3D wedge %dx%dx%d %s (%d cells), 20 degree ramp, z-periodic, %s numbering; M 3, "
+                                "AUSM + DM/Venkatakrishnan + least squares, RK4, time step mode %d, CFL 0.8" %
+                                (a.nx3, a.ny3, a.nz3, "tetrahedra" if a.tets else "hexahedra", N, a.numbering,
+                                 a.time_mode)) if three_d else
+                               ("2D wedge %dx%d quads (%d cells), jitter 10%% seed 1, %s numbering; %s + DM/MinMod + "
+                                "Green-Gauss, RK4, time step mode %d, CFL 0.8" % (a.nx, a.ny, N, a.numbering,
+                                                                               a.flux.upper(), a.time_mode)),
                    "cells": N, "faces": F, "rk_stages": 4, "precision": a.precision},
         "roofline": roofline,
         "cell_stage_updates_per_s": round(cell_its * 4, 1),
@@ -343,15 +361,15 @@ def main():
     if a.also_fp32 and a.precision == 64:
         el32, stats32, state32 = timed(32)
         ci32 = N * a.steps / el32
-        whole32 = survey_bytes_per_cell_iteration(2, 4, N, F)
+        whole32 = survey_bytes_per_cell_iteration(dim, 4, N, F)
         err = {}
-        for k, nm in enumerate(["u", "v", "p", "T", "rho"]):
+        for k, nm in enumerate(["u", "v", "w", "p", "T", "rho"] if three_d else ["u", "v", "p", "T", "rho"]):
             err[nm] = float(np.linalg.norm(state32[k, :N] - state[k, :N]) / np.linalg.norm(state[k, :N]))
         out["fp32"] = {"value": round(ci32, 1), "ms_per_step": round(el32 / a.steps * 1e3, 4),
                        "whole_step_frac": round(whole32 * ci32 / 1e9 / HBM_PEAK_GBS, 4),
                        "kernels_ms": {k: round(v[1], 4) for k, v in stats32.items()},
                        "rel_l2_vs_fp64_after_steps": err}
-    if a.cpu_iters > 0:
+    if a.cpu_iters > 0 and not three_d:
         from oracle.oracle import Oracle
         raw = case.raw()
         raw["dim"] = 2
