@@ -115,8 +115,13 @@ def run_warmstart(a):
     pa, pb = r["plain"], r["warm"]
     ok = (pa["outcome"] == 0) & (pb["outcome"] == 0)
     cells = r["cells"]
-    out = {"metric": "time-to-%g residual: NN warm vs plain (batched, held-out Mach/angle)" % a.tol,
-           "value": round(pa["wall"] / pb["wall"], 4), "unit": "x (plain wall / warm wall)", "n_gpus": 1,
+    # The headline is the ratio of solver work (cell-iterations) to reach the tolerance, plain over warm, over
+    # the instances that converge from both starts; batch wall times also include instances that never
+    # converge (they run their budget or diverge) and are reported beside it.
+    ratio = float(np.sum(cells[ok] * pa["iters64"][ok]) / max(1, np.sum(cells[ok] * pb["iters64"][ok])))
+    out = {"metric": "time-to-%g residual: NN warm vs plain (held-out Mach/angle, instances converging in both)"
+                     % a.tol,
+           "value": round(ratio, 4), "unit": "x (plain cell-iterations / warm cell-iterations)", "n_gpus": 1,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64 solver, bf16 MLP",
            "data": "synthetic",
            "config": {"workload": "learned warm start", "train": "%d solves at %dx%d (seed 4)" % (len(tr), tnx, tny),
@@ -125,10 +130,13 @@ def run_warmstart(a):
                       "train_steps": a.train_steps, "train_batch": a.train_batch},
            "plain_wall_s": round(pa["wall"], 3), "warm_wall_s": round(pb["wall"], 3),
            "both_converged": int(ok.sum()),
+           "outcomes_plain": {k: int(np.sum(pa["outcome"] == v)) for k, v in (("converged", 0), ("budget", 1),
+                                                                              ("diverged", 2))},
+           "outcomes_warm": {k: int(np.sum(pb["outcome"] == v)) for k, v in (("converged", 0), ("budget", 1),
+                                                                            ("diverged", 2))},
            "iters_plain": {"median": int(np.median(pa["iters64"][ok])), "sum": int(pa["iters64"][ok].sum())},
            "iters_warm": {"median": int(np.median(pb["iters64"][ok])), "sum": int(pb["iters64"][ok].sum())},
-           "cell_iters_ratio": round(float(np.sum(cells[ok] * pa["iters64"][ok]) /
-                                           max(1, np.sum(cells[ok] * pb["iters64"][ok]))), 4),
+           "wall_ratio_all_instances": round(pa["wall"] / pb["wall"], 4),
            "surrogate_rms_err": {"median": round(float(np.median(r["surrogate_rel_l2"][ok])), 5),
                                  "max": round(float(np.max(r["surrogate_rel_l2"][ok])), 5)},
            "predict_s": round(r["predict_s"], 3), "train_s": round(t_train, 2),
