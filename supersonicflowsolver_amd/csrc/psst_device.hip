@@ -1435,6 +1435,7 @@ struct psst_ctx {
     bool fused = false;
     TileDev td{};
     int n_tiles = 0, tile_threads = 256, tile_cells = 0;
+    std::vector<int32_t> tile_cell_begin;  // [n_tiles+1] first device cell of each tile
     int max_slots = 0, max_faces = 0;
     size_t shm_stage = 0, shm_grad = 0;
     // batched launches: per-launch work list (list == nullptr when unbatched)
@@ -1932,7 +1933,30 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                 throw std::invalid_argument("unsupported thermal boundary kind");
         }
         // ---- device numbering: interior cells along a Morton curve, ghosts unchanged ----
-        std::vector<int32_t> d2c_int = psst::morton_order(dim, N, NT, m->cell_centre, seg);
+        // typical cell spacing per axis: mean |dx_k| over interior faces whose
+        // centre-to-centre vector is dominated by axis k
+        double axis_h[3] = {0, 0, 0};
+        {
+            double sum[3] = {0, 0, 0};
+            int64_t cnt[3] = {0, 0, 0};
+            for (int f = 0; f < F; ++f) {
+                if (f0[f] >= N || f1[f] >= N) continue;
+                int kk = 0;
+                double best = -1.0;
+                for (int k = 0; k < dim; ++k) {
+                    double d = m->cell_centre[(size_t)k * NT + f0[f]] - m->cell_centre[(size_t)k * NT + f1[f]];
+                    if (m->face_shift) d -= m->face_shift[(size_t)k * F + f];
+                    if (std::abs(d) > best) {
+                        best = std::abs(d);
+                        kk = k;
+                    }
+                }
+                sum[kk] += best;
+                cnt[kk]++;
+            }
+            for (int k = 0; k < dim; ++k) axis_h[k] = cnt[k] ? sum[k] / (double)cnt[k] : 0.0;
+        }
+        std::vector<int32_t> d2c_int = psst::morton_order(dim, N, NT, m->cell_centre, seg, axis_h);
         c->d2c.resize((size_t)NT);
         std::vector<int32_t> c2d((size_t)NT);
         for (int d = 0; d < N; ++d) c->d2c[(size_t)d] = d2c_int[(size_t)d];
@@ -2117,20 +2141,38 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
         // ---- tile plan of the fused kernels ----
         const bool ordered_dt1 = (s->time_mode == PSST_DT_LOCAL1 && s->dt_ordering == 0);
         if (s->fused && !ordered_dt1) {
-            // defaults measured on the 3840x1280 wedge (HLLC, second order):
-            // FP64 512 cells / 512 threads, FP32 256 / 256
-            const int def_tile = s->precision == 64 ? 512 : 256;
-            const int tile = s->tile_cells > 0 ? s->tile_cells : def_tile;
-            if (tile < 32 || tile > 4096) throw std::invalid_argument("tile_cells must be 32..4096");
+            // Tile size: the caller's, or the first of a list that keeps the
+            // stage kernel's LDS within 80 KB (two or more workgroups per CU);
+            // the lists start at what the autotuner found fastest on the
+            // 3840x1280 wedge (FP64 512 cells / 512 threads, FP32 768 / 512).
             if (s->tile_threads != 0 && s->tile_threads != 256 && s->tile_threads != 512)
                 throw std::invalid_argument("tile_threads must be 256 or 512");
+            std::vector<int> cands;
+            if (s->tile_cells > 0) {
+                if (s->tile_cells < 32 || s->tile_cells > 4096) throw std::invalid_argument("tile_cells must be 32..4096");
+                cands = {s->tile_cells};
+            } else if (s->precision == 64) {
+                cands = {512, 384, 256, 192, 128};
+            } else {
+                cands = {768, 512, 384, 256, 192, 128};
+            }
+            const double cost_slot = (double)(dim + 3) * sb + sizeof(int), cost_face = (double)(dim + 2) * sb;
+            psst::TilePlan tp;
+            size_t shm_stage = 0, shm_grad = 0;
+            for (size_t ci = 0; ci < cands.size(); ++ci) {
+                tp = psst::plan_tiles(N, G, F, cands[ci], f0d.data(), f1d.data(), voff.data(), vface.data(),
+                                      loff.data(), lface.data(), seg, cost_slot, cost_face);
+                c->max_slots = tp.max_slots;
+                c->max_faces = tp.max_faces;
+                shm_stage = stage_shm(c.get(), sb);
+                shm_grad = grad_shm(c.get(), sb);
+                const bool last = ci + 1 == cands.size();
+                if (shm_stage <= 80 * 1024 || (last && shm_stage <= 160 * 1024)) break;
+            }
+            const int tile = tp.tile;
             c->tile_threads = s->tile_threads ? s->tile_threads : (tile >= 512 ? 512 : 256);
             c->tile_cells = tile;
-            psst::TilePlan tp = psst::plan_tiles(N, G, F, tile, f0d.data(), f1d.data(), voff.data(), vface.data(),
-                                                 loff.data(), lface.data(), seg);
-            c->max_slots = tp.max_slots;
-            c->max_faces = tp.max_faces;
-            const size_t shm_stage = stage_shm(c.get(), sb), shm_grad = grad_shm(c.get(), sb);
+            c->tile_cell_begin = tp.cell_begin;
             if (shm_stage <= 160 * 1024) {
                 const int TF = (int)tp.face_id.size();
                 std::vector<double> tfn((size_t)dim * TF), tfa((size_t)TF), trlr((size_t)dim * TF), tbeta((size_t)TF);
@@ -2871,16 +2913,21 @@ psst_status psst_batch_create(int32_t n, const psst_mesh_desc *meshes, const pss
             bt->ctx = nullptr;
             throw std::invalid_argument("the batch's tiles do not fit in LDS (use smaller tile_cells)");
         }
-        // tiles restart at every instance: ceil(N_i / tile) each, in order
+        // tiles never straddle instances: the tiles of instance i are those
+        // whose first cell lies in [cell_begin[i], cell_begin[i+1])
         bs->tile_begin.assign(1, 0);
-        std::vector<int32_t> tile_inst;
-        for (int i = 0; i < n; ++i) {
-            const int32_t Ni = meshes[i].n_cells;
-            const int32_t nt = (Ni + c->tile_cells - 1) / c->tile_cells;
-            for (int k = 0; k < nt; ++k) tile_inst.push_back(i);
-            bs->tile_begin.push_back(bs->tile_begin.back() + nt);
+        std::vector<int32_t> tile_inst((size_t)c->n_tiles);
+        for (int t = 0, i = 0; t < c->n_tiles; ++t) {
+            const int32_t cb0 = c->tile_cell_begin[(size_t)t];
+            while (cb0 >= bs->cell_begin[(size_t)i + 1]) {
+                bs->tile_begin.push_back(t);
+                ++i;
+            }
+            if (c->tile_cell_begin[(size_t)t + 1] > bs->cell_begin[(size_t)i + 1])
+                throw std::logic_error("a tile straddles two instances");
+            tile_inst[(size_t)t] = i;
         }
-        if (bs->tile_begin.back() != c->n_tiles) throw std::logic_error("batch tile count mismatch");
+        while ((int)bs->tile_begin.size() < n + 1) bs->tile_begin.push_back(c->n_tiles);
         bs->c2d.resize((size_t)NT);
         for (int64_t d = 0; d < NT; ++d) bs->c2d[(size_t)c->d2c[(size_t)d]] = (int32_t)d;
         bs->have_state.assign((size_t)n, 0);

@@ -33,7 +33,10 @@ uint64_t spread3(uint64_t x) {  // 21 bits -> every third bit of 63
 }  // namespace
 
 std::vector<int32_t> morton_order(int dim, int32_t N, int64_t NT, const double *centre,
-                                  const std::vector<int32_t> &seg) {
+                                  const std::vector<int32_t> &seg, const double *axis_h) {
+    double inv_h[3] = {1.0, 1.0, 1.0};
+    if (axis_h)
+        for (int k = 0; k < dim; ++k) inv_h[k] = axis_h[k] > 0.0 ? 1.0 / axis_h[k] : 1.0;
     std::vector<int32_t> bounds = seg.empty() ? std::vector<int32_t>{0} : seg;
     bounds.push_back(N);
     std::vector<int32_t> d2c((size_t)N);
@@ -44,7 +47,7 @@ std::vector<int32_t> morton_order(int dim, int32_t N, int64_t NT, const double *
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
         for (int k = 0; k < dim; ++k)
             for (int32_t c = b; c < e; ++c) {
-                double v = centre[(size_t)k * NT + c];
+                double v = centre[(size_t)k * NT + c] * inv_h[k];
                 lo[k] = std::min(lo[k], v);
                 hi[k] = std::max(hi[k], v);
             }
@@ -57,7 +60,7 @@ std::vector<int32_t> morton_order(int dim, int32_t N, int64_t NT, const double *
         for (int32_t c = b; c < e; ++c) {
             uint64_t q[3] = {0, 0, 0};
             for (int k = 0; k < dim; ++k) {
-                double t = (centre[(size_t)k * NT + c] - lo[k]) / span;
+                double t = (centre[(size_t)k * NT + c] * inv_h[k] - lo[k]) / span;
                 t = std::min(std::max(t, 0.0), 1.0);
                 q[k] = (uint64_t)(t * levels);
             }
@@ -71,11 +74,51 @@ std::vector<int32_t> morton_order(int dim, int32_t N, int64_t NT, const double *
     return d2c;
 }
 
+namespace {
+
+// Slots (own cells + halo) and faces of the tile [b, e).
+void tile_size(int32_t b, int32_t e, const int32_t *f0, const int32_t *f1, const int32_t *voff, const int32_t *vface,
+               std::vector<int32_t> &slot_of, std::vector<int32_t> &lface_of, std::vector<int32_t> &tile_faces,
+               std::vector<int32_t> &halo_cells) {
+    const int32_t nc = e - b;
+    for (int32_t c = b; c < e; ++c) slot_of[(size_t)c] = c - b;
+    tile_faces.clear();
+    halo_cells.clear();
+    for (int32_t c = b; c < e; ++c)
+        for (int32_t k = voff[c]; k < voff[c + 1]; ++k) {
+            int32_t f = vface[k];
+            if (lface_of[(size_t)f] < 0) {
+                lface_of[(size_t)f] = (int32_t)tile_faces.size();
+                tile_faces.push_back(f);
+            }
+        }
+    // halo slots in order of first appearance over the tile's faces
+    for (int32_t f : tile_faces)
+        for (int32_t x : {f0[f], f1[f]})
+            if (slot_of[(size_t)x] < 0) {
+                slot_of[(size_t)x] = nc + (int32_t)halo_cells.size();
+                halo_cells.push_back(x);
+            }
+}
+
+void reset(int32_t b, int32_t e, std::vector<int32_t> &slot_of, std::vector<int32_t> &lface_of,
+           const std::vector<int32_t> &tile_faces, const std::vector<int32_t> &halo_cells) {
+    for (int32_t c = b; c < e; ++c) slot_of[(size_t)c] = -1;
+    for (int32_t x : halo_cells) slot_of[(size_t)x] = -1;
+    for (int32_t f : tile_faces) lface_of[(size_t)f] = -1;
+}
+
+}  // namespace
+
 TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t *f0, const int32_t *f1,
                     const int32_t *voff, const int32_t *vface, const int32_t *loff, const int32_t *lface,
-                    const std::vector<int32_t> &seg) {
+                    const std::vector<int32_t> &seg, double cost_slot, double cost_face) {
     TilePlan p;
     p.tile = tile;
+    const int64_t NT = (int64_t)N + G;
+    std::vector<int32_t> slot_of((size_t)NT, -1);   // device cell -> local slot in the current tile
+    std::vector<int32_t> lface_of((size_t)F, -1);   // global face -> local face in the current tile
+    std::vector<int32_t> tile_faces, halo_cells;
     // tile ranges: consecutive runs of `tile` cells, restarting at every segment
     std::vector<int32_t> starts;
     {
@@ -84,8 +127,47 @@ TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t
         for (size_t sidx = 0; sidx + 1 < bounds.size(); ++sidx)
             for (int32_t c = bounds[sidx]; c < bounds[sidx + 1]; c += tile) starts.push_back(c);
     }
+    // split the tiles whose LDS cost is well above the median
+    if (cost_slot > 0.0 || cost_face > 0.0) {
+        auto cost_of = [&](int32_t b, int32_t e) {
+            tile_size(b, e, f0, f1, voff, vface, slot_of, lface_of, tile_faces, halo_cells);
+            const double c = cost_slot * (double)((e - b) + (int32_t)halo_cells.size()) + cost_face * (double)tile_faces.size();
+            reset(b, e, slot_of, lface_of, tile_faces, halo_cells);
+            return c;
+        };
+        const size_t T0 = starts.size();
+        std::vector<double> cost(T0);
+        for (size_t t = 0; t < T0; ++t) cost[t] = cost_of(starts[t], t + 1 < T0 ? starts[t + 1] : N);
+        std::vector<double> sorted = cost;
+        std::nth_element(sorted.begin(), sorted.begin() + sorted.size() / 2, sorted.end());
+        const double cap = 1.15 * sorted[sorted.size() / 2];
+        std::vector<int32_t> refined;
+        refined.reserve(T0);
+        for (size_t t = 0; t < T0; ++t) {
+            const int32_t b = starts[t], e = t + 1 < T0 ? starts[t + 1] : N;
+            if (cost[t] <= cap) {
+                refined.push_back(b);
+                continue;
+            }
+            // halve until every piece fits (or is 32 cells)
+            std::vector<std::pair<int32_t, int32_t>> stack{{b, e}};
+            std::vector<int32_t> pieces;
+            while (!stack.empty()) {
+                auto r = stack.back();
+                stack.pop_back();
+                if (r.second - r.first > 32 && cost_of(r.first, r.second) > cap) {
+                    const int32_t mid = r.first + (r.second - r.first) / 2;
+                    stack.push_back({mid, r.second});
+                    stack.push_back({r.first, mid});
+                } else {
+                    pieces.push_back(r.first);
+                }
+            }
+            refined.insert(refined.end(), pieces.begin(), pieces.end());
+        }
+        starts.swap(refined);
+    }
     p.n_tiles = (int32_t)starts.size();
-    const int64_t NT = (int64_t)N + G;
     for (int32_t c = 0; c < N; ++c) {
         p.W_visit = std::max(p.W_visit, voff[c + 1] - voff[c]);
         p.W_list = std::max(p.W_list, loff[c + 1] - loff[c]);
@@ -96,33 +178,11 @@ TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t
     p.halo_off.assign(1, 0);
     p.face_off.assign(1, 0);
     p.ghost_off.assign(1, 0);
-    std::vector<int32_t> slot_of((size_t)NT, -1);   // device cell -> local slot in the current tile
-    std::vector<int32_t> lface_of((size_t)F, -1);   // global face -> local face in the current tile
-    std::vector<int32_t> tile_faces, halo_cells;
     for (int32_t t = 0; t < p.n_tiles; ++t) {
         const int32_t b = starts[(size_t)t], e = (t + 1 < p.n_tiles) ? starts[(size_t)t + 1] : N;
         p.cell_begin[(size_t)t] = b;
         const int32_t nc = e - b;
-        for (int32_t c = b; c < e; ++c) slot_of[(size_t)c] = c - b;
-        tile_faces.clear();
-        halo_cells.clear();
-        for (int32_t c = b; c < e; ++c)
-            for (int32_t k = voff[c]; k < voff[c + 1]; ++k) {
-                int32_t f = vface[k];
-                if (lface_of[(size_t)f] < 0) {
-                    lface_of[(size_t)f] = (int32_t)tile_faces.size();
-                    tile_faces.push_back(f);
-                }
-            }
-        // halo slots in order of first appearance over the tile's faces
-        for (int32_t f : tile_faces) {
-            for (int32_t x : {f0[f], f1[f]}) {
-                if (slot_of[(size_t)x] < 0) {
-                    slot_of[(size_t)x] = nc + (int32_t)halo_cells.size();
-                    halo_cells.push_back(x);
-                }
-            }
-        }
+        tile_size(b, e, f0, f1, voff, vface, slot_of, lface_of, tile_faces, halo_cells);
         const int32_t ns = nc + (int32_t)halo_cells.size();
         const int32_t nf = (int32_t)tile_faces.size();
         if (ns >= 32768 || nf >= 16384) throw std::runtime_error("tile too large for its local indices");
@@ -157,10 +217,7 @@ TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t
         p.halo_off.push_back((int32_t)p.halo.size());
         p.face_off.push_back((int32_t)p.face_id.size());
         p.ghost_off.push_back((int32_t)p.ghost.size());
-        // reset scratch
-        for (int32_t c = b; c < e; ++c) slot_of[(size_t)c] = -1;
-        for (int32_t x : halo_cells) slot_of[(size_t)x] = -1;
-        for (int32_t f : tile_faces) lface_of[(size_t)f] = -1;
+        reset(b, e, slot_of, lface_of, tile_faces, halo_cells);
     }
     p.cell_begin[(size_t)p.n_tiles] = N;
     return p;

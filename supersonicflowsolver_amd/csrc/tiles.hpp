@@ -21,8 +21,11 @@ namespace psst {
 // seg (optional) lists the first canonical cell of each independent
 // segment (instances of a batch, seg[0] = 0): cells are ordered within their
 // segment, and segments keep their order.
+// axis_h (optional, dim values): the mesh's typical cell spacing per axis;
+// coordinates are divided by it so that Morton boxes are cubes in cells,
+// not in length (an anisotropic mesh would otherwise get slab-shaped tiles).
 std::vector<int32_t> morton_order(int dim, int32_t N, int64_t NT, const double *centre /* [dim][NT] */,
-                                  const std::vector<int32_t> &seg = {});
+                                  const std::vector<int32_t> &seg = {}, const double *axis_h = nullptr);
 
 struct TilePlan {
     int32_t tile = 256;                 // nominal cells per tile
@@ -46,8 +49,12 @@ struct TilePlan {
 // f0/f1: device cell ids per global face; voff/vface and loff/lface: per
 // device interior cell, global face ids in face-loop and face-list order.
 // seg (optional): first device cell of each segment; no tile spans two.
+// The workgroup's LDS is sized for the largest tile, so a few irregular
+// tiles (where the Morton curve jumps) would cost every workgroup: with
+// cost_slot/cost_face > 0 (LDS bytes per slot and per face), tiles whose
+// cost exceeds 1.15 x the median are halved until they fit.
 TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t *f0, const int32_t *f1,
                     const int32_t *voff, const int32_t *vface, const int32_t *loff, const int32_t *lface,
-                    const std::vector<int32_t> &seg = {});
+                    const std::vector<int32_t> &seg = {}, double cost_slot = 0.0, double cost_face = 0.0);
 
 }  // namespace psst
