@@ -337,7 +337,8 @@ def main():
     # workload; null for other configurations
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_wedge3840x1280.json")
-    if os.path.exists(tpath) and a.precision == 64 and (a.nx, a.ny, a.flux) == (3840, 1280, "hllc"):
+    if (os.path.exists(tpath) and not three_d and a.precision == 64 and (a.nx, a.ny, a.flux) == (3840, 1280, "hllc")
+            and a.numbering == "rcm"):
         with open(tpath) as f:
             tr = json.load(f).get(dom)
         if tr:

@@ -7,6 +7,8 @@
 // explicit_solve(), save_data().  Failures surface as std::runtime_error, as
 // the reference's own exceptions do.
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <memory>
@@ -51,6 +53,16 @@ psst::WedgeParams to_params(const psst_wedge_params *p) {
 
 namespace psst {
 RawMesh generate_wedge_any(const WedgeParams &p);
+
+void log_phase(const char *what) {
+    static const bool on = std::getenv("PSST_VERBOSE") != nullptr;
+    if (!on) return;
+    static auto last = std::chrono::steady_clock::now();
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "psst: %s (%.1f s)\n", what, std::chrono::duration<double>(now - last).count());
+    std::fflush(stderr);
+    last = now;
+}
 }
 
 // ---------------------------------------------------------------------------
@@ -157,8 +169,11 @@ psst_status psst_case_open_wedge(const psst_wedge_params *p, const char *case_pr
     if (!p || !case_prefix || !out) return PSST_BAD_ARG;
     try {
         auto pc = std::make_unique<psst_case>();
+        psst::log_phase("wedge mesh: generating");
         pc->raw = psst::generate_wedge_any(to_params(p));
+        psst::log_phase("wedge mesh: generated; building the grid");
         pc->c.grid = psst::build_grid(pc->raw);
+        psst::log_phase("grid built");
         psst::load_settings(pc->c, case_prefix);
         *out = pc.release();
         return PSST_OK;

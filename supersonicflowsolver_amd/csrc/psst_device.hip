@@ -37,6 +37,10 @@
 #include "psst_gpu.h"
 #include "tiles.hpp"
 
+namespace psst {
+void log_phase(const char *what);
+}
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -1956,7 +1960,9 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
             }
             for (int k = 0; k < dim; ++k) axis_h[k] = cnt[k] ? sum[k] / (double)cnt[k] : 0.0;
         }
+        psst::log_phase("context: mesh validated");
         std::vector<int32_t> d2c_int = psst::morton_order(dim, N, NT, m->cell_centre, seg, axis_h);
+        psst::log_phase("context: cells renumbered");
         c->d2c.resize((size_t)NT);
         std::vector<int32_t> c2d((size_t)NT);
         for (int d = 0; d < N; ++d) c->d2c[(size_t)d] = d2c_int[(size_t)d];
@@ -2162,6 +2168,7 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
             for (size_t ci = 0; ci < cands.size(); ++ci) {
                 tp = psst::plan_tiles(N, G, F, cands[ci], f0d.data(), f1d.data(), voff.data(), vface.data(),
                                       loff.data(), lface.data(), seg, cost_slot, cost_face);
+                psst::log_phase(("context: tile plan of " + std::to_string(cands[ci]) + " cells").c_str());
                 c->max_slots = tp.max_slots;
                 c->max_faces = tp.max_faces;
                 shm_stage = stage_shm(c.get(), sb);
@@ -2218,6 +2225,7 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                 c->fused = true;
             }
         }
+        psst::log_phase("context: geometry and tiles uploaded");
         // ---- state: sized for FP64 whatever the precision, so that
         // psst_set_precision can switch in place ----
         const size_t sb8 = sizeof(double);

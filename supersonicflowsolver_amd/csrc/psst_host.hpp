@@ -17,6 +17,12 @@
 #include "psst_gpu.h"
 
 namespace psst {
+// Phase log of long host set-ups: with PSST_VERBOSE set in the environment,
+// prints "psst: <what> (<seconds since the previous line>)" to stderr.
+void log_phase(const char *what);
+}  // namespace psst
+
+namespace psst {
 
 // ---------------------------------------------------------------------------
 // Raw mesh: the content of a Fluent ASCII .msh/.cas file.
