@@ -996,20 +996,6 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, con
     const int cb = td.cell_begin[t], nc = td.cell_begin[t + 1] - cb;
     const int hb = td.halo_off[t], ns = nc + (td.halo_off[t + 1] - hb);
     const int fb = td.face_off[t], nf = td.face_off[t + 1] - fb;
-    // prefetch the update inputs of this thread's first cell; they arrive
-    // while the slots load and the faces are computed
-    const int c_pre = cb + (int)threadIdx.x;
-    const bool has_pre = (int)threadIdx.x < nc;
-    T w0_pre[WS];
-    double V_pre = 0.0, S_pre[DIM];
-    short ve_pre[4];
-#pragma unroll
-    for (int j = 0; j < WS; ++j) w0_pre[j] = has_pre ? s.W0[(size_t)j * g.N + c_pre] : T(0);
-    V_pre = has_pre ? g.vol[c_pre] : 1.0;
-#pragma unroll
-    for (int k = 0; k < DIM; ++k) S_pre[k] = (has_pre && LC == PSST_DT_LOCAL2) ? g.S[(size_t)k * g.N + c_pre] : 0.0;
-#pragma unroll
-    for (int w = 0; w < 4; ++w) ve_pre[w] = (has_pre && w < td.Wv) ? td.visit[(size_t)w * g.N + c_pre] : (short)-1;
     stage_slots<DIM, T, ORDER == 1>(g, td, s, cb, nc, hb, ns, sq, scell);
     __syncthreads();
     // ---- faces of the tile ----
@@ -1066,12 +1052,11 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, con
     for (int j = 0; j < WS; ++j) rmax[j] = 0;
     for (int i = threadIdx.x; i < nc; i += blockDim.x) {
         const int c = cb + i;
-        const bool pre = (i == (int)threadIdx.x);
         T R[WS];
 #pragma unroll
         for (int j = 0; j < WS; ++j) R[j] = T(0);
         for (int w = 0; w < td.Wv; ++w) {
-            const int e = (pre && w < 4) ? ve_pre[w] : td.visit[(size_t)w * g.N + c];
+            const int e = td.visit[(size_t)w * g.N + c];
             if (e < 0) break;
             const int lf = e & 0x3fff;
             const bool first = (e & 0x4000) != 0;
@@ -1089,16 +1074,7 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, con
         if (LC == PSST_DT_UNIFORM) {
             dt = T(ph.dt_uniform);
         } else if (LC == PSST_DT_LOCAL2) {
-            if (pre) {
-                // dt_local2 with the prefetched projected areas
-                T cc = dsqrt(T(ph.gamma) * T(ph.rm) * Tc);
-                T den = T(0);
-#pragma unroll
-                for (int j = 0; j < DIM; ++j) den += (fabs(uc[j]) + cc) * T(S_pre[j]) * T(0.5);
-                dt = fdiv(T(ph.cfl) * T(V_pre), den);
-            } else {
-                dt = dt_local2<DIM, T>(g, c, uc, Tc, ph);
-            }
+            dt = dt_local2<DIM, T>(g, c, uc, Tc, ph);
         } else {
             // time step mode 1 with every neighbour at the stage-start state
             // (Jacobi; psstLocalTimeCalculation.cpp:90-131 otherwise)
@@ -1123,11 +1099,11 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, con
             }
             dt = fdiv(T(ph.cfl) * T(g.vol[c]), den);
         }
-        const T V = T(pre ? V_pre : g.vol[c]);
+        const T V = T(g.vol[c]);
         T w[WS];
 #pragma unroll
         for (int j = 0; j < WS; ++j) {
-            T w0 = pre ? w0_pre[j] : s.W0[(size_t)j * g.N + c];
+            T w0 = s.W0[(size_t)j * g.N + c];
             w[j] = w0 - fdiv(dt * alpha * R[j], V);
             unsigned long long ab = abs_bits(R[j]);
             rmax[j] = rmax[j] > ab ? rmax[j] : ab;
@@ -1155,8 +1131,17 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, con
     residual_max<WS>(rmax, res_row);
 }
 
+// Occupancy of the stage kernel: in 2D FP64 the compiler is asked to keep
+// five waves per SIMD (96 VGPRs), which with 224-cell tiles (about 30 KB of
+// LDS) gives five workgroups per CU; measured 15% faster than four waves
+// on the 3840x1280 wedge.  3D and FP32 keep the compiler's choice.
+template <int DIM, typename T>
+struct StageWaves {
+    static constexpr int value = (DIM == 2 && sizeof(T) == 8) ? 5 : 1;
+};
+
 template <int DIM, typename T, int FLUX, int ORDER, int LIM, int LC, int NTHR>
-__global__ void __launch_bounds__(NTHR) k_stage(Geom g, TileDev td, State<T> s, Phys ph, T alpha, int last_stage,
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(StageWaves<DIM, T>::value))) k_stage(Geom g, TileDev td, State<T> s, Phys ph, T alpha, int last_stage,
                                                   unsigned long long *res_row, const BCDev *__restrict__ bcs,
                                                   BatchDev bd) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -2149,8 +2134,8 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
         if (s->fused && !ordered_dt1) {
             // Tile size: the caller's, or the first of a list that keeps the
             // stage kernel's LDS within 80 KB (two or more workgroups per CU);
-            // the lists start at what the autotuner found fastest on the
-            // 3840x1280 wedge (FP64 512 cells / 512 threads, FP32 768 / 512).
+            // the lists start at what was fastest on the 3840x1280 wedge
+            // (FP64 224 cells / 256 threads, FP32 768 / 512).
             if (s->tile_threads != 0 && s->tile_threads != 256 && s->tile_threads != 512)
                 throw std::invalid_argument("tile_threads must be 256 or 512");
             std::vector<int> cands;
@@ -2158,7 +2143,7 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                 if (s->tile_cells < 32 || s->tile_cells > 4096) throw std::invalid_argument("tile_cells must be 32..4096");
                 cands = {s->tile_cells};
             } else if (s->precision == 64) {
-                cands = {512, 384, 256, 192, 128};
+                cands = {224, 192, 160, 128};
             } else {
                 cands = {768, 512, 384, 256, 192, 128};
             }
