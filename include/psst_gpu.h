@@ -210,9 +210,9 @@ psst_status psst_autotune(const psst_mesh_desc *mesh, const psst_settings *setti
  * the scheme (every psst_settings field but the boundary table); they are
  * stored as one union mesh whose tiles never straddle two instances.  After
  * every iteration the device computes each instance's residual norms and
- * freezes the instances that reached the tolerance; the remaining tiles are
- * served to a persistent grid from per-XCD work queues, and each iteration
- * is replayed from a captured hipGraph.
+ * freezes the instances that reached the tolerance; one workgroup runs each
+ * remaining tile, an eighth of them per XCD, and each iteration is replayed
+ * from a captured hipGraph.
  * ------------------------------------------------------------------------- */
 typedef struct psst_batch psst_batch;
 

@@ -911,42 +911,25 @@ __device__ __forceinline__ int tile_of_block(int n_tiles) {
 
 // Work list of a batched launch.  A batch holds many independent instances
 // in one union mesh; its tiles never straddle two instances.  The tiles of
-// the instances still iterating form the list [0, *count); a persistent
-// grid pops them from 8 queues, one per XCD, each a contiguous eighth of the
-// list (neighbouring tiles share halo cells, so they are best served by the
-// same L2), and steals from the other queues once its own is empty.
-// list == nullptr: one tile per workgroup, mapped by tile_of_block.
+// the instances still iterating form the list [0, *count).  The grid covers
+// every tile of the batch (a captured graph keeps its launch shape); each of
+// the 8 XCDs takes a contiguous eighth of the active list (neighbouring tiles
+// share halo cells, so they are best served by the same L2), one tile per
+// workgroup, and the workgroups beyond the active list exit at once.
+// Measured on the 4.9M-cell wedge as one instance: a persistent grid that
+// strides through the list was 30% slower, and one that popped tiles from
+// per-XCD atomic queues 3.5x slower (the pops serialise on the counters).
+// list == nullptr: unbatched, one tile per workgroup.
 struct BatchDev {
     const int *__restrict__ list;       // active tiles
     const int *__restrict__ count;      // number of active tiles
-    int *ctr;                           // [8] queue heads of this launch (zeroed before it)
     const int *__restrict__ tile_inst;  // [n_tiles] instance of each tile
     unsigned long long *res;            // [n_inst][dim+2][kResSlots] residual slots of the iteration
 };
 
-__device__ __forceinline__ int pop_item(const BatchDev &bd) {
-    __shared__ int s_item;
-    __syncthreads();  // every thread has read the previous item
-    if (threadIdx.x == 0) {
-        const int cnt = *bd.count;
-        const int per = (cnt + 7) >> 3;
-        const int x0 = (int)(blockIdx.x & 7u);
-        int item = -1;
-        for (int q = 0; q < 8 && item < 0; ++q) {
-            const int x = (x0 + q) & 7;
-            const int b = x * per, n = min(cnt, b + per) - b;
-            if (n <= 0 || __hip_atomic_load(&bd.ctr[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n) continue;
-            const int k = atomicAdd(&bd.ctr[x], 1);
-            if (k < n) item = b + k;
-        }
-        s_item = item;
-    }
-    __syncthreads();
-    return s_item;
-}
-
-// Drives a per-tile body over this workgroup's tiles: the one tile of an
-// unbatched launch, or the popped items of a batched one.
+// Runs the per-tile body on this workgroup's tile: tile_of_block of an
+// unbatched launch, or item (b & 7) * per + (b >> 3) of the active list of a
+// batched one (workgroup b runs on XCD b & 7).
 template <int WS, typename Body>
 __device__ __forceinline__ void for_each_tile(const TileDev &td, const BatchDev &bd, unsigned long long *res_row,
                                               Body body) {
@@ -955,12 +938,13 @@ __device__ __forceinline__ void for_each_tile(const TileDev &td, const BatchDev 
         if (t < td.n_tiles) body(t, res_row);
         return;
     }
-    for (;;) {
-        const int item = pop_item(bd);
-        if (item < 0) return;
-        const int t = bd.list[item];
-        body(t, bd.res + (size_t)bd.tile_inst[t] * WS * kResSlots);
-    }
+    const int cnt = *bd.count;
+    const int per = (cnt + 7) >> 3, j = (int)(blockIdx.x >> 3);
+    if (j >= per) return;
+    const int item = (int)(blockIdx.x & 7u) * per + j;
+    if (item >= cnt) return;
+    const int t = bd.list[item];
+    body(t, bd.res + (size_t)bd.tile_inst[t] * WS * kResSlots);
 }
 
 __device__ __forceinline__ int end0(unsigned e) { return (int)(e & 0x7fffu); }
@@ -1265,18 +1249,17 @@ __global__ void __launch_bounds__(NTHR) k_grad_lsq_tile(Geom g, TileDev td, Stat
 // the residual history row, the drop against the instance's reference row,
 // and the freeze of every instance that reached the tolerance (or went
 // non-finite).  When the active set changed it rebuilds the list of active
-// tiles, in instance order.  It also clears the queue heads of the next
-// iteration's launches.
+// tiles, in instance order.
 // ---------------------------------------------------------------------------
 struct BatchCheck {
-    int n, H, n_ctr;
+    int n, H;
     unsigned long long *res;  // [n][WS][kResSlots]
     double *res0;             // [n][WS]; negative: take the next row
     double *hist;             // [H][n][WS]
     double *drop;             // [n]
     int *active, *conv, *frozen_buf, *iter;
     const int *tile_begin;    // [n+1]
-    int *list, *count, *ctr;
+    int *list, *count;
 };
 
 constexpr int kCheckThreads = 1024;
@@ -1365,7 +1348,6 @@ __global__ void __launch_bounds__(kCheckThreads) k_batch_check(BatchCheck b, dou
             }
         }
     }
-    for (int k = threadIdx.x; k < 8 * b.n_ctr; k += kCheckThreads) b.ctr[k] = 0;
     __syncthreads();
     if (!rebuild_only && threadIdx.x == 0) *b.iter = it + 1;
     if (s_changed) rebuild_list(b);
@@ -1383,7 +1365,7 @@ struct BatchState {
     std::vector<int32_t> c2d;                                  // union canonical -> device
     std::vector<char> have_state, enabled;
     std::vector<int32_t> buf_of;  // state buffer holding each instance's latest state
-    int *tile_inst = nullptr, *tile_begin_dev = nullptr, *list = nullptr, *count = nullptr, *ctr = nullptr;
+    int *tile_inst = nullptr, *tile_begin_dev = nullptr, *list = nullptr, *count = nullptr;
     unsigned long long *res = nullptr;
     double *res0 = nullptr, *hist = nullptr, *drop = nullptr;
     int *active = nullptr, *conv = nullptr, *frozen_buf = nullptr, *iter = nullptr;
@@ -1560,34 +1542,15 @@ void launch_shm(psst_ctx *c, const char *name, K kernel, dim3 grid, dim3 block, 
     }
 }
 
-// Grid of a tile kernel: one workgroup per tile (8-aligned for the XCD
-// mapping), or, for a batched launch, a persistent grid sized to the
-// kernel's occupancy on every CU.
+// Grid of a tile kernel: one workgroup per tile of the context (8-aligned
+// for the XCD mapping), batched or not.
 template <typename K>
-dim3 tile_grid(psst_ctx *c, K kernel, int threads, size_t shm) {
-    if (!c->bd.list) return dim3(8 * ((c->n_tiles + 7) / 8));
-    static std::map<std::pair<const void *, size_t>, int> occ;
-    auto key = std::make_pair((const void *)kernel, shm);
-    auto it = occ.find(key);
-    if (it == occ.end()) {
-        set_shm(kernel, shm);
-        int per_cu = 0, dev = 0, n_cu = 0;
-        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, shm), "occupancy");
-        hip_check(hipGetDevice(&dev), "hipGetDevice");
-        hip_check(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
-        it = occ.emplace(key, std::max(1, per_cu) * std::max(1, n_cu)).first;
-    }
-    const int g = std::min(it->second, c->n_tiles);
-    return dim3(8 * ((g + 7) / 8));
+dim3 tile_grid(psst_ctx *c, K, int, size_t) {
+    return dim3(8 * ((c->n_tiles + 7) / 8));
 }
 
-// Work list of launch `idx` of an iteration (0: gradients, 1 + k: stage k):
-// each launch has its own queue heads.
-BatchDev launch_bd(psst_ctx *c, int idx) {
-    BatchDev b = c->bd;
-    if (b.list) b.ctr = c->bd.ctr + 8 * idx;
-    return b;
-}
+// Work list of a launch (list == nullptr: unbatched).
+BatchDev launch_bd(psst_ctx *c, int) { return c->bd; }
 
 size_t stage_shm(const psst_ctx *c, size_t sb) {
     const int NQ = c->dim + 3, NV = c->dim + 2;
@@ -2578,7 +2541,6 @@ BatchCheck batch_check_args(psst_ctx *c) {
     BatchCheck k;
     k.n = b.n;
     k.H = b.H;
-    k.n_ctr = 1 + c->s.rk_levels;
     k.res = b.res;
     k.res0 = b.res0;
     k.hist = b.hist;
@@ -2590,7 +2552,6 @@ BatchCheck batch_check_args(psst_ctx *c) {
     k.tile_begin = b.tile_begin_dev;
     k.list = b.list;
     k.count = b.count;
-    k.ctr = b.ctr;
     return k;
 }
 
@@ -2933,7 +2894,6 @@ psst_status psst_batch_create(int32_t n, const psst_mesh_desc *meshes, const pss
         bs->list = dev_alloc<int>(cx, (size_t)c->n_tiles);
         std::vector<int> zeros_i((size_t)std::max(n, 8 * 9), 0), minus1((size_t)n, -1);
         bs->count = dev_upload(cx, zeros_i.data(), 1);
-        bs->ctr = dev_upload(cx, zeros_i.data(), (size_t)8 * 9);
         bs->iter = dev_upload(cx, zeros_i.data(), 1);
         bs->active = dev_upload(cx, zeros_i.data(), (size_t)n);
         bs->frozen_buf = dev_upload(cx, zeros_i.data(), (size_t)n);
@@ -3075,7 +3035,7 @@ psst_status psst_batch_run(psst_batch *bt, int32_t max_iter, double tol, int32_t
         hip_check(hipMemcpy(b.active, active.data(), sizeof(int) * n, hipMemcpyHostToDevice), "hipMemcpy");
         hip_check(hipMemcpy(b.conv, conv.data(), sizeof(int) * n, hipMemcpyHostToDevice), "hipMemcpy");
         launch_check(c, tol, c->cur, 1);
-        c->bd = BatchDev{b.list, b.count, b.ctr, b.tile_inst, b.res};
+        c->bd = BatchDev{b.list, b.count, b.tile_inst, b.res};
         const int64_t it0 = b.iter_host;
         int done = 0;
         bool any = true, first = true;
