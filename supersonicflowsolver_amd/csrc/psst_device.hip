@@ -3129,8 +3129,8 @@ psst_status psst_autotune(const psst_mesh_desc *m, const psst_settings *s, const
     }
     // tile shapes: cells per tile x threads per workgroup (the kernels are
     // built for 256 and 512 threads)
-    static const int shapes[][2] = {{128, 256}, {192, 256}, {256, 256}, {256, 512}, {384, 256},
-                                    {384, 512}, {512, 256}, {512, 512}, {768, 512}, {1024, 512}};
+    static const int shapes[][2] = {{128, 256}, {160, 256}, {192, 256}, {224, 256}, {256, 256}, {256, 512},
+                                    {384, 256}, {384, 512}, {512, 256}, {512, 512}, {768, 512}, {1024, 512}};
     double best_ms = 1e300;
     int k_out = 0;
     best[0] = best[1] = 0;
