@@ -285,6 +285,15 @@ psst_status psst_case_raw(const psst_case *c, double *xyz, int8_t *face_nv, int6
                           int64_t *zone_bc, char *zone_names, int32_t names_len);
 /* Writes the reference's `.sav` checkpoint for a primitive state. */
 psst_status psst_case_write_sav(const psst_case *c, const double *prim, int64_t iter_value, const char *path);
+/* Writes the reference's `.dat` field file for a primitive state [(dim+3)][N+G]:
+ * p, T, rho, u, v (, w) of every interior and ghost cell in Fluent data
+ * sections (psstAnsysOutputManager::private_write_field,
+ * psstOutputManager.cpp:71-128). */
+psst_status psst_case_write_dat(const psst_case *c, const double *prim, const char *path);
+/* Writes the reference's `.cas` file of the case's mesh: vertices, faces and
+ * cells per zone, the trees of an adapted mesh, zone names
+ * (psstAnsysOutputManager::private_write_cas_file, psstOutputManager.cpp:379-398). */
+psst_status psst_case_write_cas(const psst_case *c, const char *path);
 /* Replaces the settings of a case (e.g. to change the scheme or precision). */
 psst_status psst_case_set_settings(psst_case *c, const psst_settings *settings);
 void psst_case_close(psst_case *c);
@@ -292,7 +301,8 @@ const char *psst_case_last_error(void);
 
 /* Runs a whole case the way the reference's main() does: explicit_solve for
  * the .inp iteration count on the device, writing `<prefix>_residual.dat`
- * and `<prefix>.sav` in the reference's formats (psstFiniteVolumeGrid.cpp:248-400). */
+ * and, at every save, `<prefix>.dat`, `<prefix>.cas` and `<prefix>.sav` in
+ * the reference's formats (psstFiniteVolumeGrid.cpp:240-400). */
 psst_status psst_case_solve(psst_case *c);
 
 #ifdef __cplusplus

@@ -18,7 +18,8 @@
  *   - whole-run fields: bit-identical to a g++ build of the reference
  *     (oracle/Makefile target `ref`) on the committed fixtures in
  *     tests/golden/ (80x240 HLL/RK1/lc=1 reference-held run; generated
- *     40x120 C1 runs; a small matrix of schemes);
+ *     40x120 C1 runs; a small matrix of schemes; the reference's 3D hex
+ *     meshes; its adapted meshes with parent zones and face/cell trees);
  *   - `.sav` and residual file of the reference-held run byte-compared;
  *   - limiters, Roe average, fluxes, geometry, gradients also pinned by
  *     closed forms and invariants.
@@ -163,6 +164,26 @@ void add_zone_name(RawMesh &m, long id, const std::string &name) {
     }
 }
 
+void add_zone_connection(RawMesh &m, long parent, long first, long last, long child) {
+    /* psstAnsysGridReader.cpp:655-686 add_zones_storage_connection: a parent
+     * zone seen first here is created as a parent face zone (type 31) with
+     * the tree's range; a child zone seen first here gets an empty storage.
+     * Both keep that position in the zone order when their own section
+     * arrives later and overwrites range and type. */
+    if (m.zone_pos.find(parent) == m.zone_pos.end()) {
+        RawZone z;
+        z.id = parent; z.first = first; z.last = last; z.bc = 31;
+        m.zone_pos[parent] = m.zones.size();
+        m.zones.push_back(z);
+    }
+    if (m.zone_pos.find(child) == m.zone_pos.end()) {
+        RawZone z;
+        z.id = child;
+        m.zone_pos[child] = m.zones.size();
+        m.zones.push_back(z);
+    }
+}
+
 RawMesh read_mesh(const std::string &path) {
     /* Semantics of psstAnsysDirector::build_step and psstAnsysBuilder::read_*
      * (psstAnsysGridReader.cpp:948-1116, :1138-1306); a plain recursive
@@ -250,7 +271,14 @@ RawMesh read_mesh(const std::string &path) {
             c.skip_to_close();
             add_zone_name(m, id, name);
         } else if (sec == 58 || sec == 59) {
-            throw std::runtime_error("adapted meshes (face/cell trees) are not supported by the oracle");
+            /* read_cell_tree / read_face_tree (:1068-1116): the parent-child
+             * links themselves are not used by the solver; only the zone
+             * storages they create (add_zones_storage_connection, :655-686)
+             * matter, through the file order of zones */
+            std::vector<long> h = c.header_hex();
+            if (h.size() < 4) throw std::runtime_error("bad tree section header");
+            add_zone_connection(m, h[2], h[0], h[1], h[3]);
+            c.skip_to_close();
         } else {
             c.skip_to_close();
         }
@@ -541,7 +569,6 @@ void build_grid(Oracle &o, const RawMesh &m) {
     /* face-volume links in file zone order (:825-883) */
     for (const RawZone &z : m.zones) {
         if (z.bc == 1 || z.bc == 32) continue;
-        if (z.bc == 31) throw std::runtime_error("parent face zones are not supported by the oracle");
         ZoneStore &fs = o.face_zones.at(z.id);
         for (size_t j = 0; j < fs.items.size(); ++j) {
             const RawFace &rf = m.faces[z.first + j - 1];
@@ -549,6 +576,18 @@ void build_grid(Oracle &o, const RawMesh &m) {
             for (int k = 0; k < 2; ++k) {
                 long cell = rf.c[k];
                 int vi;
+                if (z.bc == 31) {
+                    /* parent faces (:860-874): neighbours linked, but the
+                     * face joins no cell's face list and no ghost exists */
+                    if (cell == 0) {
+                        f.nb[k] = -1;
+                        continue;
+                    }
+                    long vz = m.cell_zone.at(cell - 1);
+                    const RawZone &cz = m.zones[m.zone_pos.at(vz)];
+                    f.nb[k] = o.vol_zones.at(vz).items.at(cell - cz.first);
+                    continue;
+                }
                 if (cell == 0) {
                     vi = o.vol_zones.at(z.id).items.at(j);
                 } else {

@@ -158,6 +158,16 @@ class Case:
         prim = np.ascontiguousarray(prim, dtype=np.float64)
         _check(lib().psst_case_write_sav(self._h, _dp(prim), int(iter_value), str(path).encode()))
 
+    def write_dat(self, prim, path):
+        """The reference's `.dat` field file (p, T, rho, velocity of every interior and ghost cell)."""
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        assert prim.shape == (self.dim + 3, self.n_total)
+        _check(lib().psst_case_write_dat(self._h, _dp(prim), str(path).encode()))
+
+    def write_cas(self, path):
+        """The reference's `.cas` file of the mesh (zones, faces, cells, trees of an adapted mesh)."""
+        _check(lib().psst_case_write_cas(self._h, str(path).encode()))
+
     def raw(self):
         """The mesh as read: vertices, faces, cells and zones (file order)."""
         L = lib()

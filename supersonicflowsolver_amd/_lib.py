@@ -86,6 +86,8 @@ _SIGS = {
                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_char_p,
                                 C.c_int32]),
     "psst_case_write_sav": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int64, C.c_char_p]),
+    "psst_case_write_dat": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_char_p]),
+    "psst_case_write_cas": (C.c_int, [C.c_void_p, C.c_char_p]),
     "psst_case_solve": (C.c_int, [C.c_void_p]),
     "psst_case_close": (None, [C.c_void_p]),
     "psst_case_last_error": (C.c_char_p, []),

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpsst.so")
 
-HOST_SOURCES = ["mesh_io.cpp", "grid.cpp", "setup.cpp", "meshgen.cpp", "host_api.cpp", "tiles.cpp"]
+HOST_SOURCES = ["mesh_io.cpp", "grid.cpp", "setup.cpp", "output.cpp", "meshgen.cpp", "host_api.cpp", "tiles.cpp"]
 DEVICE_SOURCES = ["psst_device.hip", "mlp.hip"]
 
 # -ffp-contract=off: every multiply and add is rounded separately, as in the

@@ -6,6 +6,9 @@
 // psstAnsysGridReader.cpp:688-907) and why:
 //   * one ghost cell per boundary face, its centre = the face centre, V = 0
 //     (:770-781, psstFiniteVolumeGridTools.cpp:725-729);
+//   * adapted meshes: parent cells (zone type 32) and parent faces (type 31)
+//     take no part in the solve; a parent face joins no cell's face list and
+//     has no ghost (:860-874), and the flux and gradient loops skip it;
 //   * a cell's face list is built by visiting face zones in FILE order, then
 //     faces in zone order (:825-883).  The time-step and least-squares sums run
 //     over this list, so its order decides their rounding;
@@ -126,7 +129,7 @@ Grid build_grid(const RawMesh &m) {
     }
     for (GridZone &z : g.zones) {
         const RawZone &rz = *by_id[z.id];
-        if (z.bc != 1 && z.bc != 2 && z.bc != 12) {
+        if (z.bc != 1 && z.bc != 2 && z.bc != 12 && z.bc != 31 && z.bc != 32) {
             z.has_cells = true;
             z.cell_begin = n_int + n_gh;
             z.cell_count = (int32_t)(rz.last - rz.first + 1);
@@ -136,7 +139,10 @@ Grid build_grid(const RawMesh &m) {
     g.N = n_int;
     g.G = n_gh;
     const int64_t NT = (int64_t)n_int + n_gh;
-    if (n_int != m.n_cells()) throw std::runtime_error("mesh: cells outside fluid zones");
+    int64_t n_parent_cells = 0;
+    for (const GridZone &z : g.zones)
+        if (z.bc == 32) n_parent_cells += by_id[z.id]->last - by_id[z.id]->first + 1;
+    if (n_int + n_parent_cells != m.n_cells()) throw std::runtime_error("mesh: cells outside fluid zones");
     // file cell index (1-based) -> canonical
     std::vector<int32_t> canon_cell((size_t)m.n_cells(), -1);
     for (const GridZone &z : g.zones) {
@@ -146,8 +152,13 @@ Grid build_grid(const RawMesh &m) {
     }
     // ---- canonical faces: face zones ascending id ----
     int32_t nf = 0;
+    int64_t n_parent_faces = 0;
     for (GridZone &z : g.zones) {
-        if (z.bc == 1) continue;
+        if (z.bc == 1 || z.bc == 32) continue;
+        if (z.bc == 31) {
+            n_parent_faces += by_id[z.id]->last - by_id[z.id]->first + 1;
+            continue;
+        }
         const RawZone &rz = *by_id[z.id];
         z.has_faces = true;
         z.face_begin = nf;
@@ -155,8 +166,9 @@ Grid build_grid(const RawMesh &m) {
         nf += z.face_count;
     }
     g.F = nf;
-    if (nf != m.n_faces()) throw std::runtime_error("mesh: faces outside face zones");
-    std::vector<int32_t> canon_face((size_t)nf);  // file face (0-based) -> canonical
+    if (nf + n_parent_faces != m.n_faces()) throw std::runtime_error("mesh: faces outside face zones");
+    // file face (0-based) -> canonical; -1 for parent faces
+    std::vector<int32_t> canon_face((size_t)m.n_faces(), -1);
     std::vector<int32_t> face_local((size_t)nf);  // canonical -> zone-local index
     for (const GridZone &z : g.zones) {
         if (!z.has_faces) continue;
@@ -189,6 +201,7 @@ Grid build_grid(const RawMesh &m) {
                 } else {
                     if (cell < 1 || cell > m.n_cells()) throw std::runtime_error("mesh: face cell index out of range");
                     ci = canon_cell[(size_t)cell - 1];
+                    if (ci < 0) throw std::runtime_error("mesh: active face next to a parent cell");
                 }
                 (k == 0 ? g.f0 : g.f1)[(size_t)cf] = ci;
             }
@@ -215,7 +228,7 @@ Grid build_grid(const RawMesh &m) {
     {
         std::vector<int32_t> fill(g.list_off.begin(), g.list_off.end() - 1);
         for (const RawZone &rz : m.zones) {
-            if (rz.bc == 1 || (rz.first == 0 && rz.last == 0)) continue;
+            if (rz.bc == 1 || rz.bc == 31 || rz.bc == 32 || (rz.first == 0 && rz.last == 0)) continue;
             for (int64_t i = rz.first; i <= rz.last; ++i) {
                 int32_t cf = canon_face[(size_t)i - 1];
                 g.list_face[(size_t)fill[(size_t)g.f0[(size_t)cf]]++] = cf;
@@ -244,6 +257,7 @@ Grid build_grid(const RawMesh &m) {
     g.fc.assign(3 * (size_t)nf, 0.0);
     g.fa.assign((size_t)nf, 0.0);
     for (int64_t fi = 0; fi < m.n_faces(); ++fi) {
+        if (canon_face[(size_t)fi] < 0) continue;
         FaceGeom fg = face_geometry(m, fi);
         size_t cf = (size_t)canon_face[(size_t)fi];
         for (int k = 0; k < 3; ++k) {
@@ -277,7 +291,8 @@ Grid build_grid(const RawMesh &m) {
     g.vol.assign((size_t)NT, 0.0);
     g.cx.assign(3 * (size_t)NT, 0.0);
     std::vector<size_t> file_face_of((size_t)nf);
-    for (int64_t fi = 0; fi < m.n_faces(); ++fi) file_face_of[(size_t)canon_face[(size_t)fi]] = (size_t)fi;
+    for (int64_t fi = 0; fi < m.n_faces(); ++fi)
+        if (canon_face[(size_t)fi] >= 0) file_face_of[(size_t)canon_face[(size_t)fi]] = (size_t)fi;
     // vertex j of face cf as cell c sees it (a periodic face translated by
     // +P for its cell(1)); FC likewise for the face centre
     auto off = [&](int32_t cf, int64_t c, int k) {

@@ -70,7 +70,7 @@ void log_phase(const char *what) {
 // ---------------------------------------------------------------------------
 class psstGridFV {
    public:
-    explicit psstGridFV(psst::Case &c) : case_(c) {}
+    psstGridFV(psst::Case &c, const psst::RawMesh &raw) : case_(c), raw_(raw) {}
     ~psstGridFV() {
         if (ctx_) psst_ctx_destroy(ctx_);
     }
@@ -122,9 +122,13 @@ class psstGridFV {
         out.close();
         save_data(case_.iter);
     }
+    // psstGridFV::save_data (psstFiniteVolumeGrid.cpp:240-246): the output
+    // manager's field and case files, then the checkpoint
     void save_data(int64_t iter_value) {
         std::vector<double> prim(case_.prim.size());
         check(psst_get_state(ctx_, prim.data()));
+        psst::write_dat(raw_, case_.grid, prim.data(), case_.prefix + ".dat");
+        psst::write_cas(raw_, case_.prefix + ".cas");
         psst::write_sav(case_, prim.data(), iter_value, case_.prefix + ".sav");
     }
 
@@ -133,6 +137,7 @@ class psstGridFV {
         if (st != PSST_OK) throw std::runtime_error(psst_last_error(ctx_));
     }
     psst::Case &case_;
+    const psst::RawMesh &raw_;
     psst_ctx *ctx_ = nullptr;
 };
 
@@ -275,10 +280,32 @@ psst_status psst_case_write_sav(const psst_case *c, const double *prim, int64_t 
     }
 }
 
+psst_status psst_case_write_dat(const psst_case *c, const double *prim, const char *path) {
+    if (!c || !prim || !path) return PSST_BAD_ARG;
+    try {
+        psst::write_dat(c->raw, c->c.grid, prim, path);
+        return PSST_OK;
+    } catch (const std::exception &e) {
+        g_case_error = e.what();
+        return PSST_IO_ERROR;
+    }
+}
+
+psst_status psst_case_write_cas(const psst_case *c, const char *path) {
+    if (!c || !path) return PSST_BAD_ARG;
+    try {
+        psst::write_cas(c->raw, path);
+        return PSST_OK;
+    } catch (const std::exception &e) {
+        g_case_error = e.what();
+        return PSST_IO_ERROR;
+    }
+}
+
 psst_status psst_case_solve(psst_case *c) {
     if (!c) return PSST_BAD_ARG;
     try {
-        psstGridFV fv(c->c);
+        psstGridFV fv(c->c, c->raw);
         fv.solver_parameters();
         fv.explicit_solve();
         return PSST_OK;

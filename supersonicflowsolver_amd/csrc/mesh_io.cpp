@@ -7,7 +7,9 @@
 //   * a face line is [type if the zone is mixed] v0 v1 [v2 v3] c0 c1 and the
 //     first cell index becomes cell(0), the second cell(1)
 //     (read_faces :1019-1026 with psstSimpleFace's (left, right) swap :177-194);
-//   * zones are kept in order of first appearance (add_zones_storage :600-632).
+//   * zones are kept in order of first appearance (add_zones_storage :600-632),
+//     a tree section counting as an appearance of its parent and child zones
+//     (add_zones_storage_connection :655-686).
 // Unlike the reference it parses by structure rather than by lines, so it is
 // indifferent to CRLF (quirk Q12) and to where the opening '(' of a data
 // block sits.
@@ -128,6 +130,37 @@ void add_zone(RawMesh &m, std::map<int64_t, size_t> &pos, int64_t id, int64_t fi
     }
 }
 
+// add_zones_storage_connection (psstAnsysGridReader.cpp:655-686): a parent
+// zone first seen in a tree section is created as a parent face zone (type
+// 31) with the tree's range, a child zone first seen there as an empty zone;
+// both keep that place in the zone order when their own section arrives.
+void add_zone_link(RawMesh &m, std::map<int64_t, size_t> &pos, int64_t parent, int64_t first, int64_t last,
+                   int64_t child) {
+    auto it = pos.find(parent);
+    if (it == pos.end()) {
+        RawZone z;
+        z.id = parent;
+        z.first = first;
+        z.last = last;
+        z.bc = 31;
+        z.child_zone = child;
+        pos[parent] = m.zones.size();
+        m.zones.push_back(z);
+    } else {
+        m.zones[it->second].child_zone = child;
+    }
+    it = pos.find(child);
+    if (it == pos.end()) {
+        RawZone z;
+        z.parent_zone = parent;
+        pos[child] = m.zones.size();
+        m.zones.push_back(z);
+        m.zones.back().id = child;
+    } else {
+        m.zones[it->second].parent_zone = parent;
+    }
+}
+
 }  // namespace
 
 RawMesh read_fluent_msh(const std::string &path) {
@@ -182,14 +215,19 @@ RawMesh read_fluent_msh(const std::string &path) {
                 continue;
             }
             if (h.size() < 5) throw std::runtime_error("mesh: bad cell section");
-            if (h[3] == 32) throw std::runtime_error("mesh: adapted meshes (parent cell zones) are not supported");
             add_zone(m, pos, h[0], h[1], h[2], h[3]);
-            if (m.n_cells() < h[2]) m.cell_zone.resize((size_t)h[2], 0);
-            for (int64_t i = h[1]; i <= h[2]; ++i) m.cell_zone[(size_t)i - 1] = h[0];
+            if (m.n_cells() < h[2]) {
+                m.cell_zone.resize((size_t)h[2], 0);
+                m.cell_type.resize((size_t)h[2], 0);
+            }
+            for (int64_t i = h[1]; i <= h[2]; ++i) {
+                m.cell_zone[(size_t)i - 1] = h[0];
+                m.cell_type[(size_t)i - 1] = (int8_t)h[4];
+            }
             if (s.peek('(')) {
                 ++s.p;
                 if (h[4] == 0)
-                    for (int64_t i = h[1]; i <= h[2]; ++i) s.hex();
+                    for (int64_t i = h[1]; i <= h[2]; ++i) m.cell_type[(size_t)i - 1] = (int8_t)s.hex();
                 s.expect(')');
             }
             s.skip_close();
@@ -201,24 +239,29 @@ RawMesh read_fluent_msh(const std::string &path) {
                 continue;
             }
             if (h.size() < 5) throw std::runtime_error("mesh: bad face section");
-            if (h[3] == 31) throw std::runtime_error("mesh: adapted meshes (parent face zones) are not supported");
             add_zone(m, pos, h[0], h[1], h[2], h[3]);
-            if (m.n_faces() != h[1] - 1) throw std::runtime_error("mesh: face sections out of order");
-            int64_t n = h[2] - h[1] + 1;
+            // Faces go to their declared indices, so zone sections may come in
+            // any order (the reference's own .cas writer emits them by zone
+            // id); the reference appends them in reading order instead.
+            if (h[1] < 1 || h[2] < h[1]) throw std::runtime_error("mesh: bad face section");
+            if (m.n_faces() < h[2]) {
+                m.face_nv.resize((size_t)h[2], 0);
+                m.face_v.resize(4 * (size_t)h[2], 0);
+                m.face_c.resize(2 * (size_t)h[2], 0);
+                m.face_zone.resize((size_t)h[2], 0);
+            }
             s.expect('(');
-            m.face_nv.reserve(m.face_nv.size() + (size_t)n);
-            for (int64_t i = 0; i < n; ++i) {
+            for (int64_t i = h[1] - 1; i < h[2]; ++i) {
                 int64_t t = h[4];
                 if (t == 0) t = s.hex();
                 if (t < 2 || t > 4) throw std::runtime_error("Error: Unknown face type!");
-                int64_t v[4] = {0, 0, 0, 0};
+                if (m.face_nv[(size_t)i] != 0) throw std::runtime_error("mesh: face listed twice");
+                int64_t *v = &m.face_v[4 * (size_t)i];
                 for (int k = 0; k < (int)t; ++k) v[k] = s.hex();
-                int64_t c0 = s.hex(), c1 = s.hex();
-                m.face_nv.push_back((int8_t)t);
-                m.face_v.insert(m.face_v.end(), v, v + 4);
-                m.face_c.push_back(c0);
-                m.face_c.push_back(c1);
-                m.face_zone.push_back(h[0]);
+                m.face_c[2 * (size_t)i] = s.hex();
+                m.face_c[2 * (size_t)i + 1] = s.hex();
+                m.face_nv[(size_t)i] = (int8_t)t;
+                m.face_zone[(size_t)i] = h[0];
             }
             s.expect(')');
             s.skip_close();
@@ -240,7 +283,21 @@ RawMesh read_fluent_msh(const std::string &path) {
             m.zones[it->second].kind = kind;
             m.zones[it->second].name = name;
         } else if (sec == 58 || sec == 59) {
-            throw std::runtime_error("mesh: adapted meshes (face/cell trees) are not supported");
+            // (58|59 (first last parent-zone child-zone)( n kid... per parent ))
+            // (read_cell_tree / read_face_tree, psstAnsysGridReader.cpp:1068-1116)
+            std::vector<int64_t> h = s.header();
+            if (h.size() < 4 || h[0] < 1 || h[1] < h[0]) throw std::runtime_error("mesh: bad tree section");
+            add_zone_link(m, pos, h[2], h[0], h[1], h[3]);
+            std::vector<std::vector<int64_t>> &kids = (sec == 58) ? m.cell_kids : m.face_kids;
+            if (kids.size() < (size_t)h[1]) kids.resize((size_t)h[1]);
+            s.expect('(');
+            for (int64_t i = h[0]; i <= h[1]; ++i) {
+                int64_t n = s.hex();
+                if (n < 0 || n > (1 << 20)) throw std::runtime_error("mesh: bad tree section");
+                for (int64_t j = 0; j < n; ++j) kids[(size_t)i - 1].push_back(s.hex());
+            }
+            s.expect(')');
+            s.skip_close();
         } else {
             s.skip_close();
         }
@@ -249,10 +306,14 @@ RawMesh read_fluent_msh(const std::string &path) {
     if (m.dim != 2 && m.dim != 3) throw std::runtime_error("mesh: dimension missing");
     for (int64_t z : m.cell_zone)
         if (z == 0) throw std::runtime_error("mesh: cell without zone");
+    for (int8_t nv : m.face_nv)
+        if (nv == 0) throw std::runtime_error("mesh: face index without a face");
     return m;
 }
 
 void write_fluent_msh(const RawMesh &m, const std::string &path) {
+    for (const RawZone &z : m.zones)
+        if (z.bc == 31 || z.bc == 32) throw std::runtime_error("mesh writer: adapted meshes are not supported");
     FILE *f = std::fopen(path.c_str(), "wb");
     if (!f) throw std::runtime_error("cannot write " + path);
     std::vector<char> buf(1 << 22);
@@ -266,6 +327,7 @@ void write_fluent_msh(const RawMesh &m, const std::string &path) {
         // the header is what the reference reader consumes for a header that
         // does not end with '(' (psstAnsysGridReader.cpp:1191-1203)
         int etype = (m.dim == 2) ? 3 : (m.face_nv.empty() || m.face_nv[0] == 4 ? 4 : 2);
+        if (!m.cell_type.empty()) etype = m.cell_type[(size_t)z.first - 1];
         std::fprintf(f, "(12 (%llx %llx %llx 1 %x))\n\n", (unsigned long long)z.id, (unsigned long long)z.first,
                      (unsigned long long)z.last, etype);
     }

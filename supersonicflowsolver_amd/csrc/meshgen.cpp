@@ -188,6 +188,7 @@ RawMesh generate_wedge(const WedgeParams &p) {
     }
     auto cid = [&](int i, int j) { return newid[(size_t)((int64_t)i * ny + j)]; };
     m.cell_zone.assign((size_t)NC, 2);
+    m.cell_type.assign((size_t)NC, 3);  // quadrilaterals
     auto add_face = [&](int64_t v0, int64_t v1, int64_t c0, int64_t c1, int64_t zone) {
         m.face_nv.push_back(2);
         int64_t v[4] = {v0, v1, 0, 0};
@@ -350,6 +351,7 @@ RawMesh generate_wedge_3d(const WedgeParams &p) {
     const bool tets = p.tets != 0;
     const int64_t NH = (int64_t)nx * ny * nz, NC = tets ? 6 * NH : NH;
     m.cell_zone.assign((size_t)NC, 2);
+    m.cell_type.assign((size_t)NC, tets ? 2 : 4);  // tetrahedra or hexahedra
     auto hcm = [&](int i, int j, int k) { return ((int64_t)i * ny + j) * nz + k; };
     // 1-based file id of a hex, or of tetrahedron sigma of a hex
     auto hex_cell = [&](int i, int j, int k) { return hpos[(size_t)hcm(i, j, k)] + 1; };

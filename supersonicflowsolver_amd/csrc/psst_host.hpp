@@ -30,9 +30,11 @@ namespace psst {
 struct RawZone {
     int64_t id = 0;
     int64_t first = 0, last = 0;  // 1-based index range of the zone's faces or cells
-    int64_t bc = 0;               // Fluent zone type: 1 fluid cells, 2 interior faces, others boundary
+    int64_t bc = 0;               // Fluent zone type: 1 fluid cells, 2 interior faces, 31/32 parent
+                                  // faces/cells of an adapted mesh, others boundary
     std::string kind;             // type word of the 39/45 section ("wall", "fluid", ...)
     std::string name;             // zone name used by the .bvp/.btq files
+    int64_t parent_zone = 0, child_zone = 0;  // links of the 58/59 tree sections
 };
 
 struct RawMesh {
@@ -43,7 +45,11 @@ struct RawMesh {
     std::vector<int64_t> face_c;     // 2 per face: cell(0), cell(1) as in the file; 0 = none
     std::vector<int64_t> face_zone;  // zone id per face
     std::vector<int64_t> cell_zone;  // zone id per cell
+    std::vector<int8_t> cell_type;   // Fluent element type per cell (1 tri, 2 tet, 3 quad, 4 hex, ...)
     std::vector<RawZone> zones;      // order of first appearance in the file
+    // adapted meshes: children of each parent face / cell (1-based ids), in
+    // the order the 58/59 sections list them; empty for other meshes
+    std::vector<std::vector<int64_t>> face_kids, cell_kids;
 
     int64_t n_vertices() const { return (int64_t)xyz.size() / 3; }
     int64_t n_faces() const { return (int64_t)face_nv.size(); }
@@ -51,9 +57,9 @@ struct RawMesh {
     const RawZone *zone(int64_t id) const;
 };
 
-// Fluent ASCII reader: sections 0, 2, 10, 12, 13, 39, 45 in any order, LF or
-// CRLF line ends, hex indices.  Adapted meshes (sections 58/59, zone types
-// 31/32) are rejected with an error.
+// Fluent ASCII reader: sections 0, 2, 10, 12, 13, 39, 45, 58, 59 in any
+// order, LF or CRLF line ends, hex indices.  Adapted meshes keep their parent
+// zones (types 31/32) and trees; the solver uses only the active elements.
 RawMesh read_fluent_msh(const std::string &path);
 // Writer in the layout the reference reader accepts (header lines end with
 // "(", LF line ends, vertices as %.20e).
@@ -140,6 +146,10 @@ void load_settings(Case &c, const std::string &prefix);
 void write_sav(const Case &c, const double *prim, int64_t iter_value, const std::string &path);
 // one line of `<case>_residual.dat` (psstFiniteVolumeGrid.cpp:369-396)
 std::string residual_line(int64_t iter, double seconds, const double *res, int n);
+// `.dat` field file of a primitive state [(dim+3)][N+G] (psstOutputManager.cpp:71-128)
+void write_dat(const RawMesh &m, const Grid &g, const double *prim, const std::string &path);
+// `.cas` mesh file (psstOutputManager.cpp:130-398)
+void write_cas(const RawMesh &m, const std::string &path);
 
 // ---------------------------------------------------------------------------
 // Mesh descriptor view (pointers into a Grid) for psst_ctx_create.

@@ -246,7 +246,7 @@ void load_settings(Case &cs, const std::string &prefix) {
     }
     for (size_t zi = 0; zi < g.zones.size(); ++zi) {
         const GridZone &z = g.zones[zi];
-        if (z.bc == 1 || z.bc == 2 || z.bc == 12) continue;
+        if (z.bc == 1 || z.bc == 2 || z.bc == 12 || z.bc == 31 || z.bc == 32) continue;
         if (!has_dyn[zi] || !has_th[zi])
             throw std::runtime_error("Error: boundary zone '" + z.name + "' needs both a .bvp and a .btq entry");
     }

@@ -16,6 +16,10 @@ Fixture kinds:
                      the reference's quad2D.msh, 12 iterations each.
   r3d_*              3D: the reference's wedge3D.msh and quad3D.msh hex meshes
                      (stored once under _meshes/), Green-Gauss schemes.
+  a_*                adapted (hanging-node) meshes: the reference's wedge3.msh,
+                     wedge5.msh and wedge_adaptation.msh (Fluent 16 files with
+                     parent zones and face/cell trees; stored once under
+                     _meshes/), a matrix of schemes, 12 iterations each.
 
 Every expected output comes from running the reference's own sources
 (built by `make -C oracle ref`), never from this package's code.
@@ -68,6 +72,17 @@ def strip_time(lines):
     return out
 
 
+def sha256(data):
+    import hashlib
+    return hashlib.sha256(data).hexdigest()
+
+
+def output_digests(workdir, prefix):
+    """SHA-256 of the reference's own `.dat` and `.cas` outputs of a run (its
+    final save_data), LF line ends."""
+    return {ext: sha256(lf(os.path.join(workdir, prefix + "." + ext))) for ext in ("dat", "cas")}
+
+
 def save_expected(case_dir, vals, dim, res_lines, meta):
     prim = vals.reshape(-1, dim + 3).T.copy()
     np.savez_compressed(os.path.join(case_dir, "expected.npz"), prim=prim,
@@ -97,6 +112,43 @@ def ref_wedge80x240():
             f.write("\n".join(inp))
         vals, res = run_reference(w, "wedge80_240.msh", "wedge")
     save_expected(d, vals, 2, res, {"iterations": 100, "source": "reference build, wedge.inp with maxIter=100"})
+    committed_output_digests()
+
+
+def fixture_output_digests():
+    """Adds reference_output_sha256 to fixtures made before it was recorded:
+    re-runs the reference build on the fixture's committed inputs (same mesh,
+    same settings files, so the same run) and updates meta.json only."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    from _fixtures import unpack
+    for name in sorted(os.listdir(HERE)):
+        mp = os.path.join(HERE, name, "meta.json")
+        if name == "ref_wedge80x240" or not os.path.exists(mp):
+            continue
+        with open(mp) as f:
+            meta = json.load(f)
+        if "reference_output_sha256" in meta:
+            continue
+        with tempfile.TemporaryDirectory() as w:
+            mesh, prefix, exp, _ = unpack(name, w)
+            vals, _ = run_reference(w, os.path.basename(mesh), os.path.basename(prefix))
+            if not np.array_equal(vals.reshape(exp["prim"].shape[::-1]).T, exp["prim"]):
+                raise RuntimeError(name + ": the re-run does not reproduce the committed state")
+            meta["reference_output_sha256"] = output_digests(w, os.path.basename(prefix))
+        with open(mp, "w") as f:
+            json.dump(meta, f, indent=1, sort_keys=True)
+
+
+def committed_output_digests():
+    """The reference's committed wedge.dat / wedge.cas (its 3000-iteration run
+    on wedge80_240.msh) as SHA-256 digests of their LF text, in the
+    ref_wedge80x240 fixture's meta.json."""
+    d = os.path.join(HERE, "ref_wedge80x240")
+    with open(os.path.join(d, "meta.json")) as f:
+        meta = json.load(f)
+    meta["committed_output_sha256"] = {ext: sha256(lf(os.path.join(REF, "wedge." + ext))) for ext in ("dat", "cas")}
+    with open(os.path.join(d, "meta.json"), "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
 
 
 def generated_case(name, params, case_kw, iterations, mesh_src=None, shared_mesh=None):
@@ -127,8 +179,11 @@ def generated_case(name, params, case_kw, iterations, mesh_src=None, shared_mesh
         for ext in ("inp", "bvp", "btq", "par", "prop"):
             shutil.copy(os.path.join(w, "case." + ext), d)
         vals, res = run_reference(w, "mesh.msh", "case")
+        digests = output_digests(w, "case")
     meta = {"iterations": iterations, "wedge_params": params, "case": kw,
-            "mesh_source": mesh_src or "supersonicflowsolver_amd wedge generator"}
+            "mesh_source": ("reference " + os.path.relpath(mesh_src, REF)) if mesh_src
+            else "supersonicflowsolver_amd wedge generator",
+            "reference_output_sha256": digests}
     if shared_mesh:
         meta["mesh"] = shared_mesh
     save_expected(d, vals, dim, res, meta)
@@ -180,11 +235,40 @@ def three_d():
         generated_case(name, None, kw, iters, mesh_src=os.path.join(REF, mesh + ".msh"), shared_mesh=mesh)
 
 
+# Adapted meshes: cells with up to eight faces, face-zone file order
+# (18, 16, 4, 8, 3, 2, parents...) unlike the ascending map order, so the
+# time-step and least-squares sums (file order) and the flux and Green-Gauss
+# loops (map order) visit faces differently.
+ADAPTED_CASE = dict(wall_names=("symm-head", "symm", "wedge"), mach=2.5, save_step=100000)
+ADAPTED = [
+    # name, mesh, flux, order, grad, limiter, time_mode, rk, cfl
+    ("a_wedge3_hllc_o2_gg_mm_lc2_rk4", "wedge3", 3, 2, 1, 2, 2, 4, 0.6),
+    ("a_wedge3_hll_o1_lc1_rk1", "wedge3", 2, 1, 1, 3, 1, 1, 0.6),
+    ("a_wedge3_ausm_o2_lsq_va_lc0_rk3", "wedge3", 1, 2, 2, 1, 0, 3, 0.6),
+    ("a_wedge5_ausm_o2_lsq_vl_lc1_rk3", "wedge5", 1, 2, 2, 3, 1, 3, 0.6),
+    ("a_wedge5_hllc_o2_gg_va_lc0_rk2", "wedge5", 3, 2, 1, 1, 0, 2, 0.6),
+    ("a_adapt_hll_o2_lsq_mm_lc2_rk4", "wedge_adaptation", 2, 2, 2, 2, 2, 4, 0.6),
+    ("a_adapt_hllc_o2_gg_vl_lc1_rk5", "wedge_adaptation", 3, 2, 1, 3, 1, 5, 0.6),
+]
+
+
+def adapted():
+    for name, mesh, flux, order, grad, lim, lc, rk, cfl in ADAPTED:
+        kw = dict(ADAPTED_CASE, cfl=cfl, rk_levels=rk, flux=flux, order=order, grad=grad, limiter=lim,
+                  time_mode=lc)
+        generated_case(name, None, kw, 12, mesh_src=os.path.join(REF, mesh + ".msh"), shared_mesh=mesh)
+
+
 def main():
     if not os.path.exists(REF_DUMP):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     if sys.argv[1:] == ["3d"]:
         return three_d()
+    if sys.argv[1:] == ["adapted"]:
+        return adapted()
+    if sys.argv[1:] == ["digests"]:
+        committed_output_digests()
+        return fixture_output_digests()
     ref_wedge80x240()
     for lc in (1, 2):
         generated_case("c1_lc%d" % lc, C1_MESH, dict(C1_CASE, time_mode=lc), 2000)
@@ -195,6 +279,7 @@ def main():
     generated_case("refmesh40x120_lc2", None, dict(C1_CASE, time_mode=2), 200,
                    mesh_src=os.path.join(REF, "wedge40_120.msh"))
     three_d()
+    adapted()
 
 
 if __name__ == "__main__":

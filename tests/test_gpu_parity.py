@@ -1,16 +1,22 @@
 """HIP path (libpsst.so, through the C-ABI) against the oracle and the golden
 fixtures.  FP64 results must be bit-identical: the kernels keep the
 reference's operation order and summation order.  Needs an MI355X."""
+import hashlib
 import os
 
 import numpy as np
 import pytest
 
 import supersonicflowsolver_amd as ss
-from _fixtures import GENERATED_CASES, MATRIX_CASES, THREE_D_CASES, residual_lines, unpack
+from _fixtures import ADAPTED_CASES, GENERATED_CASES, MATRIX_CASES, THREE_D_CASES, residual_lines, unpack
 from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
+
+
+def _sha(path):
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
 
 
 def _first_mismatch(a, b):
@@ -32,7 +38,8 @@ def _perturbed(prim, seed=11, amp=0.02):
 
 @pytest.mark.parametrize("name", ["c1_lc2", "m_hll_o2_lsq_mm_lc2_rk3", "m_ausm_o2_gg_va_lc2_rk5",
                                   "m_hllc_o2_lsq_va_lc1_rk4", "r3d_wedge_hllc_o2_gg_mm_lc2_rk4",
-                                  "r3d_quad_hllc_o2_gg_va_lc2_rk4"])
+                                  "r3d_quad_hllc_o2_gg_va_lc2_rk4", "a_wedge3_hllc_o2_gg_mm_lc2_rk4",
+                                  "a_wedge5_ausm_o2_lsq_vl_lc1_rk3", "a_adapt_hll_o2_lsq_mm_lc2_rk4"])
 def test_kernel_parity_gradients_and_residual(tmp_path, name):
     """One gradient pass and one flux pass on a perturbed state: bit-identical
     to the oracle (Green-Gauss / least squares, DM + limiter, Riemann flux,
@@ -51,7 +58,7 @@ def test_kernel_parity_gradients_and_residual(tmp_path, name):
 
 
 @pytest.mark.parametrize("fused", [1, 0])
-@pytest.mark.parametrize("name", GENERATED_CASES + MATRIX_CASES + THREE_D_CASES)
+@pytest.mark.parametrize("name", GENERATED_CASES + MATRIX_CASES + THREE_D_CASES + ADAPTED_CASES)
 def test_full_run_matches_reference_build(tmp_path, name, fused):
     """Whole run (fixture iteration count): final state bit-identical to the
     reference build's, residual history bit-identical to the oracle's and equal
@@ -69,6 +76,9 @@ def test_full_run_matches_reference_build(tmp_path, name, fused):
     assert residual_lines(res) == exp["residual"]
     o = Oracle(mesh, prefix)
     assert np.array_equal(res, o.run(n))
+    # the field file of the device state equals the one the reference build wrote
+    case.write_dat(st, str(tmp_path / "gpu.dat"))
+    assert _sha(str(tmp_path / "gpu.dat")) == meta["reference_output_sha256"]["dat"]
 
 
 def test_reference_golden_run_80x240(tmp_path):
@@ -83,6 +93,12 @@ def test_reference_golden_run_80x240(tmp_path):
     case.write_sav(st, 3001, str(tmp_path / "gpu.sav"))
     with open(str(tmp_path / "gpu.sav")) as f, open(prefix + ".sav") as g:
         assert f.read() == g.read()
+    # .dat (every interior and ghost cell, density included) and .cas equal
+    # the reference's committed wedge.dat and wedge.cas
+    case.write_dat(st, str(tmp_path / "gpu.dat"))
+    case.write_cas(str(tmp_path / "gpu.cas"))
+    assert _sha(str(tmp_path / "gpu.dat")) == meta["committed_output_sha256"]["dat"]
+    assert _sha(str(tmp_path / "gpu.cas")) == meta["committed_output_sha256"]["cas"]
     with open(prefix + "_residual.dat") as f:
         ref = []
         for l in f:
@@ -104,11 +120,12 @@ def test_chunked_runs_equal_one_run(tmp_path):
     assert np.array_equal(a, b) and np.array_equal(r1, r2)
 
 
-def test_case_solve_writes_reference_files(tmp_path):
-    """psst_case_solve (the reference's main() on the device) writes the same
-    residual rows and .sav as the reference's committed run (200 iterations of
-    the golden setup, compared with the oracle CLI semantics)."""
-    mesh, prefix, exp, meta = unpack("m_hllc_o1_lc1_rk4", str(tmp_path))
+@pytest.mark.parametrize("name", ["m_hllc_o1_lc1_rk4", "a_wedge5_hllc_o2_gg_va_lc0_rk2"])
+def test_case_solve_writes_reference_files(tmp_path, name):
+    """psst_case_solve (the reference's main() on the device) writes the
+    residual rows and .sav of the oracle's run of the fixture, and the .dat and
+    .cas the reference build wrote for it."""
+    mesh, prefix, exp, meta = unpack(name, str(tmp_path))
     case = ss.Case.open(mesh, prefix)
     case.solve()
     o = Oracle(mesh, prefix)
@@ -122,6 +139,9 @@ def test_case_solve_writes_reference_files(tmp_path):
     o.write_sav(str(tmp_path / "o.sav"), meta["iterations"] + 1)
     with open(prefix + ".sav") as f, open(str(tmp_path / "o.sav")) as g:
         assert f.read() == g.read()
+    # save_data also wrote the field and case files, as the reference's does
+    assert _sha(prefix + ".dat") == meta["reference_output_sha256"]["dat"]
+    assert _sha(prefix + ".cas") == meta["reference_output_sha256"]["cas"]
 
 
 def test_jacobi_time_step_differs_but_converges(tmp_path):

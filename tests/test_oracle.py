@@ -8,7 +8,9 @@ Pins:
     bit-identical, residual files equal, for the C1 setups (HLLC, DM/MinMod,
     Green-Gauss, RK4, lc 1 and 2, 2000 iterations) and a matrix of schemes
     (HLL/HLLC/AUSM x order 1/2 x GG/LSQ x three limiters x lc 0/1/2 x RK1..5),
-    and 3D hexahedral runs on the reference's wedge3D.msh and quad3D.msh;
+    3D hexahedral runs on the reference's wedge3D.msh and quad3D.msh, and
+    runs on its adapted (hanging-node) meshes wedge3.msh, wedge5.msh and
+    wedge_adaptation.msh;
   * freestream preservation, linear exactness of the gradients, limiter
     closed forms -- properties that do not come from the reference at all.
 """
@@ -17,7 +19,7 @@ import os
 import numpy as np
 import pytest
 
-from _fixtures import GENERATED_CASES, MATRIX_CASES, THREE_D_CASES, residual_lines, unpack
+from _fixtures import ADAPTED_CASES, GENERATED_CASES, MATRIX_CASES, THREE_D_CASES, residual_lines, unpack
 from oracle.oracle import Oracle
 
 
@@ -51,7 +53,7 @@ def test_reference_build_dump_80x240_100it(tmp_path):
     assert residual_lines(res) == exp["residual"]
 
 
-@pytest.mark.parametrize("name", GENERATED_CASES + MATRIX_CASES + THREE_D_CASES)
+@pytest.mark.parametrize("name", GENERATED_CASES + MATRIX_CASES + THREE_D_CASES + ADAPTED_CASES)
 def test_reference_build_dumps(tmp_path, name):
     mesh, prefix, exp, meta = unpack(name, str(tmp_path))
     o = Oracle(mesh, prefix)
