@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -1402,7 +1403,12 @@ struct BatchState {
     int64_t iter_host = 0;
     bool w_dirty = false;
     bool use_graph = true;
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    // Tile kernels of a run launch one workgroup per tile of the batch's
+    // active-list capacity: all tiles at first, then the smallest power of
+    // two (at least 64) that holds the active list when a chunk starts; a
+    // captured graph per capacity and state-buffer parity.
+    int grid_tiles = 0;  // 0: every tile of the batch
+    std::map<int, std::array<hipGraphExec_t, 2>> exec;
     int graph_precision = 0;
     double graph_tol = -1.0;
 };
@@ -1576,7 +1582,15 @@ void launch_shm(psst_ctx *c, const char *name, K kernel, dim3 grid, dim3 block, 
 // for the XCD mapping), batched or not.
 template <typename K>
 dim3 tile_grid(psst_ctx *c, K, int, size_t) {
-    return dim3(8 * ((c->n_tiles + 7) / 8));
+    const int tiles = (c->batch && c->batch->grid_tiles > 0) ? c->batch->grid_tiles : c->n_tiles;
+    return dim3(8 * ((tiles + 7) / 8));
+}
+
+// Launch capacity of a batched run for `count` active tiles.
+int batch_grid_tiles(int count, int n_tiles) {
+    int cap = 64;
+    while (cap < count && cap < n_tiles) cap <<= 1;
+    return std::min(cap, n_tiles);
 }
 
 // Work list of a launch (list == nullptr: unbatched).
@@ -2125,10 +2139,13 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
         // ---- tile plan of the fused kernels ----
         const bool ordered_dt1 = (s->time_mode == PSST_DT_LOCAL1 && s->dt_ordering == 0);
         if (s->fused && !ordered_dt1) {
-            // Tile size: the caller's, or the first of a list that keeps the
-            // stage kernel's LDS within 80 KB (two or more workgroups per CU);
-            // the lists start at what was fastest on the 3840x1280 wedge
-            // (FP64 224 cells / 256 threads, FP32 768 / 512).
+            // Tile size: the caller's, or the first of a list whose stage
+            // kernel LDS lets a CU (160 KB) hold 16 or more waves.  The lists
+            // start at what was fastest on the 3840x1280 wedge (FP64 224 cells
+            // / 256 threads, FP32 768 / 512); a 3D hexahedral tile carries
+            // about as many halo cells as own cells, and there 128-cell FP64
+            // tiles (four workgroups per CU) ran 31% faster than 224-cell ones
+            // (two).
             if (s->tile_threads != 0 && s->tile_threads != 256 && s->tile_threads != 512)
                 throw std::invalid_argument("tile_threads must be 256 or 512");
             std::vector<int> cands;
@@ -2136,7 +2153,7 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                 if (s->tile_cells < 32 || s->tile_cells > 4096) throw std::invalid_argument("tile_cells must be 32..4096");
                 cands = {s->tile_cells};
             } else if (s->precision == 64) {
-                cands = {224, 192, 160, 128};
+                cands = {224, 192, 160, 128, 96};
             } else {
                 cands = {768, 512, 384, 256, 192, 128};
             }
@@ -2146,13 +2163,16 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
             for (size_t ci = 0; ci < cands.size(); ++ci) {
                 tp = psst::plan_tiles(N, G, F, cands[ci], f0d.data(), f1d.data(), voff.data(), vface.data(),
                                       loff.data(), lface.data(), seg, cost_slot, cost_face);
-                psst::log_phase(("context: tile plan of " + std::to_string(cands[ci]) + " cells").c_str());
                 c->max_slots = tp.max_slots;
                 c->max_faces = tp.max_faces;
                 shm_stage = stage_shm(c.get(), sb);
                 shm_grad = grad_shm(c.get(), sb);
+                psst::log_phase(("context: tile plan of " + std::to_string(cands[ci]) + " cells, stage LDS " +
+                                 std::to_string(shm_stage) + " B").c_str());
                 const bool last = ci + 1 == cands.size();
-                if (shm_stage <= 80 * 1024 || (last && shm_stage <= 160 * 1024)) break;
+                const int thr = s->tile_threads ? s->tile_threads : (cands[ci] >= 512 ? 512 : 256);
+                const size_t waves = (160 * 1024 / std::max<size_t>(shm_stage, 1)) * (size_t)(thr / 64);
+                if (waves >= 16 || (last && shm_stage <= 160 * 1024)) break;
             }
             const int tile = tp.tile;
             c->tile_threads = s->tile_threads ? s->tile_threads : (tile >= 512 ? 512 : 256);
@@ -2539,8 +2559,9 @@ void psst_ctx_destroy(psst_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->batch)
-        for (hipGraphExec_t &e : c->batch->exec)
-            if (e) (void)hipGraphExecDestroy(e);
+        for (auto &kv : c->batch->exec)
+            for (hipGraphExec_t &e : kv.second)
+                if (e) (void)hipGraphExecDestroy(e);
     for (auto &p : c->pending) {
         (void)hipEventDestroy(p.second.first);
         (void)hipEventDestroy(p.second.second);
@@ -2560,10 +2581,10 @@ namespace {
 
 void destroy_graphs(psst_ctx *c) {
     if (!c->batch) return;
-    for (hipGraphExec_t &e : c->batch->exec) {
-        if (e) (void)hipGraphExecDestroy(e);
-        e = nullptr;
-    }
+    for (auto &kv : c->batch->exec)
+        for (hipGraphExec_t &e : kv.second)
+            if (e) (void)hipGraphExecDestroy(e);
+    c->batch->exec.clear();
 }
 
 BatchCheck batch_check_args(psst_ctx *c) {
@@ -2610,7 +2631,7 @@ void batch_graph_iteration(psst_ctx *c, double tol) {
         b.graph_tol = tol;
     }
     const int parity = c->cur;
-    hipGraphExec_t &ex = b.exec[parity];
+    hipGraphExec_t &ex = b.exec[b.grid_tiles][parity];
     if (!ex) {
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
@@ -3065,6 +3086,12 @@ psst_status psst_batch_run(psst_batch *bt, int32_t max_iter, double tol, int32_t
         hip_check(hipMemcpy(b.active, active.data(), sizeof(int) * n, hipMemcpyHostToDevice), "hipMemcpy");
         hip_check(hipMemcpy(b.conv, conv.data(), sizeof(int) * n, hipMemcpyHostToDevice), "hipMemcpy");
         launch_check(c, tol, c->cur, 1);
+        int count = 0;
+        hip_check(hipMemcpyAsync(&count, b.count, sizeof(int), hipMemcpyDeviceToHost, c->stream), "hipMemcpy");
+        hip_check(hipStreamSynchronize(c->stream), "psst_batch_run");
+        // the active list only shrinks during a run, so a capacity chosen at
+        // a chunk's start holds it for the whole chunk
+        b.grid_tiles = batch_grid_tiles(count, c->n_tiles);
         c->bd = BatchDev{b.list, b.count, b.tile_inst, b.res};
         const int64_t it0 = b.iter_host;
         int done = 0;
@@ -3085,7 +3112,9 @@ psst_status psst_batch_run(psst_batch *bt, int32_t max_iter, double tol, int32_t
             }
             hip_check(hipMemcpyAsync(active.data(), b.active, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream),
                       "hipMemcpy");
+            hip_check(hipMemcpyAsync(&count, b.count, sizeof(int), hipMemcpyDeviceToHost, c->stream), "hipMemcpy");
             hip_check(hipStreamSynchronize(c->stream), "psst_batch_run");
+            b.grid_tiles = batch_grid_tiles(count, c->n_tiles);
             if (c->timing) flush_timing(c);
             if (res_hist)
                 for (int k = 0; k < m; ++k) {
@@ -3099,6 +3128,7 @@ psst_status psst_batch_run(psst_batch *bt, int32_t max_iter, double tol, int32_t
             for (int i = 0; i < n; ++i) any = any || active[(size_t)i];
         }
         c->bd = BatchDev{};
+        b.grid_tiles = 0;
         std::vector<int> frozen((size_t)n);
         std::vector<double> drop((size_t)n);
         hip_check(hipMemcpy(conv.data(), b.conv, sizeof(int) * n, hipMemcpyDeviceToHost), "hipMemcpy");
@@ -3123,6 +3153,7 @@ psst_status psst_batch_run(psst_batch *bt, int32_t max_iter, double tol, int32_t
         return worst;
     } catch (const std::exception &e) {
         c->bd = BatchDev{};
+        b.grid_tiles = 0;
         bt->err = e.what();
         return PSST_HIP_ERROR;
     }

@@ -1,8 +1,9 @@
 """Batched solves (psst_batch_*): every instance of a batch must be bitwise
 identical, in FP64, to the same case run alone through psst_ctx (which is
 itself bitwise identical to the oracle): same residual rows, same stopping
-iteration, same final state.  Covers the persistent work-queue launch, the
-per-instance freeze at the tolerance, hipGraph replay against eager launches,
+iteration, same final state.  Covers the launch of one workgroup per active
+tile (and its capacity shrinking with the active list), the per-instance
+freeze at the tolerance, hipGraph replay against eager launches,
 odd stage counts (alternating state buffers), the FP32 -> FP64 switch and a
 non-finite instance.  Needs an MI355X."""
 import numpy as np
@@ -59,6 +60,21 @@ def test_batch_tolerance_freeze_bitwise(tmp_path):
         out = b.run(400, tol=tol, history=True, allow_no_convergence=True)
         assert len(set(out["iters"].tolist())) > 1, out["iters"]
         _compare(b, cases, out, 400, tol)
+
+
+def test_batch_launch_capacity_follows_the_active_list(tmp_path):
+    """32-cell tiles: about 110 tiles, of which the 60x20 instance holds a
+    third.  Instances that stop within the first 256-iteration chunk shrink the
+    active list, so later chunks launch a smaller grid from graphs captured for
+    that capacity; every instance stays bitwise equal to its lone run."""
+    sizes = [(36, 12, 3.8, 6.0), (60, 20, 1.8, 21.0), (45, 15, 3.5, 8.0), (24, 8, 3.8, 6.0), (30, 10, 3.1, 9.0)]
+    cases = _cases(tmp_path, sizes)
+    tol = 2e-2
+    with ss.Batch(cases, tile_cells=32, tile_threads=256) as b:
+        out = b.run(900, tol=tol, history=True, allow_no_convergence=True)
+        print("iterations", out["iters"].tolist(), "tiles", b.info()["tiles"], flush=True)
+        assert min(out["iters"]) < 256 < max(out["iters"]), out["iters"]
+        _compare(b, cases, out, 900, tol, tile_cells=32, tile_threads=256)
 
 
 def test_batch_odd_stage_count_and_continuation(tmp_path):
