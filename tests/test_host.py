@@ -120,6 +120,59 @@ def test_crlf_mesh_is_accepted(tmp_path):
     assert np.array_equal(a["xyz"], b["xyz"]) and np.array_equal(a["face_v"], b["face_v"])
 
 
+def _face_sections(text):
+    """Spans of the face-zone sections "(13 (id first last bc type)( ... ))"
+    of a mesh written by the package's generator (header zone 0 excluded)."""
+    spans = []
+    pos = 0
+    while True:
+        a = text.find("(13 (", pos)
+        if a < 0:
+            return spans
+        head = text[a + 5:text.index(")", a)].split()
+        b = text.index(")\n)\n", a) + 4 if head[0] != "0" else text.index("\n", a) + 1
+        if head[0] != "0":
+            spans.append((a, b))
+        pos = b
+
+
+def test_face_sections_in_any_order(tmp_path):
+    """Face sections are placed at their declared indices: a file whose face
+    zones are not in index order (as the reference's own .cas writer emits
+    them) reads to the same faces."""
+    mesh, prefix, _, _ = unpack("m_hll_o1_lc0_rk1", str(tmp_path))
+    with open(mesh) as f:
+        text = f.read()
+    spans = _face_sections(text)
+    assert len(spans) >= 3
+    parts = [text[a:b] for a, b in spans]
+    shuffled = text[:spans[0][0]] + "".join(parts[::-1]) + text[spans[-1][1]:]
+    assert shuffled != text
+    with open(mesh + ".shuffled", "w") as f:
+        f.write(shuffled)
+    a = ss.Case.open(mesh, prefix).raw()
+    b = ss.Case.open(mesh + ".shuffled", prefix).raw()
+    for k in ("xyz", "face_nv", "face_v", "face_c", "face_zone", "cell_zone"):
+        assert np.array_equal(a[k], b[k]), k
+    assert sorted(a["zone_id"].tolist()) == sorted(b["zone_id"].tolist())
+
+
+def test_inconsistent_face_sections_raise(tmp_path):
+    mesh, prefix, _, _ = unpack("m_hll_o1_lc0_rk1", str(tmp_path))
+    with open(mesh) as f:
+        text = f.read()
+    spans = _face_sections(text)
+    a, b = spans[1]
+    with open(mesh + ".twice", "w") as f:
+        f.write(text[:b] + text[a:b] + text[b:])
+    with pytest.raises(ss.PsstError, match="listed twice"):
+        ss.Case.open(mesh + ".twice", prefix)
+    with open(mesh + ".gap", "w") as f:
+        f.write(text[:a] + text[b:])
+    with pytest.raises(ss.PsstError, match="without a face|outside"):
+        ss.Case.open(mesh + ".gap", prefix)
+
+
 def test_sweep_generator_and_draws():
     """splitmix64 first output for seed 0 is Vigna's published value; the
     instance draw is deterministic and inside its ranges."""
