@@ -210,6 +210,7 @@ __global__ void k_outer_part(int M, int chunk, int mi, int nj, const TA_ *__rest
     const int i = e / nj, j = e % nj;
     const int m0 = blockIdx.x * chunk, m1 = min(M, m0 + chunk);
     float acc = 0.f;
+#pragma unroll 8
     for (int m = m0; m < m1; ++m) acc += (a ? (float)a[(size_t)m * lda + i] : 1.0f) * (float)b[(size_t)m * ldb + j];
     part[(size_t)blockIdx.x * mi * nj + e] = acc;
 }
@@ -219,6 +220,7 @@ __global__ void k_reduce(int S, int count, const float *__restrict__ part, float
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= count) return;
     float acc = 0.f;
+#pragma unroll 8
     for (int s = 0; s < S; ++s) acc += part[(size_t)s * count + e];
     out[e] = acc;
 }
@@ -379,9 +381,14 @@ struct psst_mlp {
     }
 
     // split reduction of a(m,i) b(m,j) over M rows into g[dst .. dst + mi*nj)
+    // Rows per partial sum of outer(): 32 rows give M/32 workgroups, enough
+    // to fill the GPU at the training batch sizes (256 rows left 64
+    // workgroups, each walking 256 dependent loads: 58 us per call at
+    // M = 16384, 4 us with 32 rows).
+    static constexpr int kOuterChunk = 32;
     template <typename TA_, typename TB_>
     void outer(int M, int mi, int nj, const TA_ *a, int lda, const TB_ *b, int ldb, int64_t dst) {
-        const int chunk = 256, S = (M + chunk - 1) / chunk;
+        const int chunk = kOuterChunk, S = (M + chunk - 1) / chunk;
         const int count = mi * nj;
         hipLaunchKernelGGL((k_outer_part<TA_, TB_>), dim3(S, nblk((size_t)count)), dim3(kThreads), 0, stream, M,
                            chunk, mi, nj, a, lda, b, ldb, part);
@@ -433,7 +440,10 @@ struct psst_mlp {
     }
 
     void ensure_part(int M) {
-        const size_t need = std::max<size_t>((size_t)64 * W * W, (size_t)((M + 255) / 256) * W * std::max(W, 8));
+        // split-K partials of the hidden dW, and outer()'s partials (at most
+        // W * max(n_in, n_out) entries per row chunk)
+        const size_t need = std::max<size_t>((size_t)64 * W * W, (size_t)((M + kOuterChunk - 1) / kOuterChunk) * W *
+                                                                     std::max(std::max(n_in, n_out), 1));
         if (need <= part_cap) return;
         release(part);
         part = alloc<float>(need);
