@@ -32,6 +32,7 @@
 #include <map>
 #include <memory>
 #include <stdexcept>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -2099,6 +2100,32 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
             c->level_off_dev = dev_upload(c.get(), c->level_off.data(), c->level_off.size());
         }
         // ---- device geometry ----
+        // Device face order: faces sorted by their lower device cell id, so
+        // that the face-indexed kernels (unfused fluxes, Green-Gauss and
+        // least-squares sweeps, the reference-ordered time step) read
+        // neighbouring cells in neighbouring lanes.  Every order-dependent
+        // sum walks a per-cell face list, which keeps the reference's order
+        // whatever the face numbering.  The tile plan works on the canonical
+        // host arrays and is not affected.
+        std::vector<int32_t> forder((size_t)F), fpos((size_t)F);
+        for (int f = 0; f < F; ++f) forder[(size_t)f] = f;
+        std::stable_sort(forder.begin(), forder.end(), [&](int32_t a, int32_t b) {
+            return std::min(f0d[(size_t)a], f1d[(size_t)a]) < std::min(f0d[(size_t)b], f1d[(size_t)b]);
+        });
+        for (int p = 0; p < F; ++p) fpos[(size_t)forder[(size_t)p]] = p;
+        // [rows][F] arrays in device face order, and face ids as device positions
+        auto upload_by_face = [&](const auto *src, size_t rows) {
+            using V = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
+            std::vector<V> out(rows * (size_t)F);
+            for (size_t r = 0; r < rows; ++r)
+                for (int p = 0; p < F; ++p) out[r * F + p] = src[r * F + (size_t)forder[(size_t)p]];
+            return dev_upload(c.get(), out.data(), out.size());
+        };
+        auto upload_positions = [&](const int32_t *ids, size_t n) {
+            std::vector<int32_t> out(n);
+            for (size_t i = 0; i < n; ++i) out[i] = fpos[(size_t)ids[i]];
+            return dev_upload(c.get(), out.data(), out.size());
+        };
         Geom &g = c->geom;
         g.N = N;
         g.G = G;
@@ -2106,20 +2133,20 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
         g.F = F;
         g.vol = dev_upload(c.get(), vol_d.data(), vol_d.size());
         g.cx = dev_upload(c.get(), cx_d.data(), cx_d.size());
-        g.fn = dev_upload(c.get(), fn3.data(), fn3.size());
-        g.fa = dev_upload(c.get(), m->face_area, (size_t)F);
-        g.beta = dev_upload(c.get(), m->face_beta, (size_t)F);
-        g.f0 = dev_upload(c.get(), f0d.data(), (size_t)F);
-        g.f1 = dev_upload(c.get(), f1d.data(), (size_t)F);
+        g.fn = upload_by_face(fn3.data(), 3);
+        g.fa = upload_by_face(m->face_area, 1);
+        g.beta = upload_by_face(m->face_beta, 1);
+        g.f0 = upload_by_face(f0d.data(), 1);
+        g.f1 = upload_by_face(f1d.data(), 1);
+        g.rlr = upload_by_face(rlr3.data(), 3);
+        g.vface = upload_positions(vface.data(), vface.size());
+        g.lface = upload_positions(lface.data(), lface.size());
+        g.gface = upload_positions(m->ghost_face, (size_t)G);
         g.voff = dev_upload(c.get(), voff.data(), voff.size());
-        g.vface = dev_upload(c.get(), vface.data(), vface.size());
         g.loff = dev_upload(c.get(), loff.data(), loff.size());
-        g.lface = dev_upload(c.get(), lface.data(), lface.size());
         g.lnb = dev_upload(c.get(), lnb.data(), lnb.size());
         g.lsqc = dev_upload(c.get(), lsqc.data(), lsqc.size());
-        g.rlr = dev_upload(c.get(), rlr3.data(), rlr3.size());
         g.S = dev_upload(c.get(), S.data(), S.size());
-        g.gface = dev_upload(c.get(), m->ghost_face, (size_t)G);
         g.gnb = dev_upload(c.get(), gnb.data(), (size_t)G);
         g.gbc = dev_upload(c.get(), m->ghost_bc, (size_t)G);
         {
