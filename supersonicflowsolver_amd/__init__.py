@@ -51,13 +51,15 @@ def write_wedge_msh(params, path):
 def write_case_files(prefix, *, cfl=0.8, time_mode=2, iterations=2000, save_step=5000, rk_levels=4,
                      rk_custom=None, flux=3, order=2, grad=1, limiter=2, mach=2.5, p=101325.0, T=300.0,
                      molar_mass=0.029, cp=None, init_mode=0, bottom_bc="15", wall_names=("symm", "wedge", "wall", "up"),
-                     dim=2, in_name="in", out_name="out"):
+                     dim=2, in_name="in", out_name="out", out_p=None, wall_T=None):
     """Writes the five settings files of a wedge case in the reference's formats
     (psstFiniteVolumeGrid.cpp:36-194, :403-717).  gamma = 1.4 is realised through
     cp = 3.5 R/M with the reference's R = 8.31 (quirk Q10); the inflow speed is
     mach * sqrt(gamma rm T) along x (dim velocity components in 3D).
     .bvp/.btq are written without a trailing newline (the reference re-reads
-    the last line otherwise, quirk Q11)."""
+    the last line otherwise, quirk Q11).  The outlet is supersonic (26), or
+    subsonic with fixed pressure out_p (25); the walls are adiabatic (18), or
+    held at temperature wall_T (11)."""
     R = 8.31
     rm = R / molar_mass
     if cp is None:
@@ -84,11 +86,11 @@ def write_case_files(prefix, *, cfl=0.8, time_mode=2, iterations=2000, save_step
     zeros = "\t0.0" * (dim - 1)
     lines = ["%s\t\t23\t%r\t%s\t%r\t//u,v,p" % (in_name, float(u), zeros, float(p))]
     lines += ["%s\t\t%s" % (n, bottom_bc) for n in wall_names]
-    lines += ["%s\t\t26" % out_name]
+    lines += ["%s\t\t26" % out_name if out_p is None else "%s\t\t25\t%r\t//p" % (out_name, float(out_p))]
     with open(prefix + ".bvp", "w") as f:
         f.write("\n".join(lines))
-    lines = (["%s\t\t27\t%r\t//T" % (in_name, float(T))] + ["%s\t\t18" % n for n in wall_names] +
-             ["%s\t\t25" % out_name])
+    walls = ["%s\t\t18" % n if wall_T is None else "%s\t\t11\t%r\t//T" % (n, float(wall_T)) for n in wall_names]
+    lines = ["%s\t\t27\t%r\t//T" % (in_name, float(T))] + walls + ["%s\t\t25" % out_name]
     with open(prefix + ".btq", "w") as f:
         f.write("\n".join(lines))
     with open(prefix + ".par", "w") as f:

@@ -12,6 +12,8 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 GENERATED_CASES = ["c1_lc1", "c1_lc2", "refmesh40x120_lc2"]
 MATRIX_CASES = sorted(d for d in (os.listdir(GOLDEN) if os.path.isdir(GOLDEN) else []) if d.startswith("m_"))
 THREE_D_CASES = sorted(d for d in (os.listdir(GOLDEN) if os.path.isdir(GOLDEN) else []) if d.startswith("r3d_"))
+# subsonic outlet (25) and isothermal walls (11)
+BC_CASES = sorted(d for d in (os.listdir(GOLDEN) if os.path.isdir(GOLDEN) else []) if d.startswith("b_"))
 # adapted (hanging-node) meshes of the reference: parent zones, face/cell trees
 ADAPTED_CASES = sorted(d for d in (os.listdir(GOLDEN) if os.path.isdir(GOLDEN) else []) if d.startswith("a_"))
 

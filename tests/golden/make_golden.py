@@ -16,6 +16,8 @@ Fixture kinds:
                      the reference's quad2D.msh, 12 iterations each.
   r3d_*              3D: the reference's wedge3D.msh and quad3D.msh hex meshes
                      (stored once under _meshes/), Green-Gauss schemes.
+  b_*                the subsonic outlet (25) and isothermal walls (11) on the
+                     10x30 wedge, three schemes.
   a_*                adapted (hanging-node) meshes: the reference's wedge3.msh,
                      wedge5.msh and wedge_adaptation.msh (Fluent 16 files with
                      parent zones and face/cell trees; stored once under
@@ -252,6 +254,26 @@ ADAPTED = [
 ]
 
 
+# Boundary conditions the other fixtures leave out: the subsonic outlet (.bvp
+# 25, fixed pressure) and the isothermal wall (.btq 11, fixed temperature).
+# (At 150 kPa and 350 K the reference's HLL/HLLC throw within 12 iterations;
+# AUSM with the uniform time step goes non-finite.)
+BC_CASES = [
+    # name, flux, order, grad, limiter, time_mode, rk
+    ("b_hll_o1_lc1_rk2_out25_wall11", 2, 1, 1, 3, 1, 2),
+    ("b_hll_o2_lsq_va_lc2_rk4_out25_wall11", 2, 2, 2, 1, 2, 4),
+    ("b_hllc_o2_lsq_va_lc2_rk4_out25_wall11", 3, 2, 2, 1, 2, 4),
+    ("b_ausm_o2_gg_mm_lc1_rk4_out25_wall11", 1, 2, 1, 2, 1, 4),
+]
+
+
+def bc_cases():
+    for name, flux, order, grad, lim, lc, rk in BC_CASES:
+        generated_case(name, SMALL, dict(cfl=0.6, rk_levels=rk, flux=flux, order=order, grad=grad, limiter=lim,
+                                         time_mode=lc, mach=2.5, save_step=100000, out_p=120000.0, wall_T=330.0),
+                       12)
+
+
 def adapted():
     for name, mesh, flux, order, grad, lim, lc, rk, cfl in ADAPTED:
         kw = dict(ADAPTED_CASE, cfl=cfl, rk_levels=rk, flux=flux, order=order, grad=grad, limiter=lim,
@@ -266,6 +288,8 @@ def main():
         return three_d()
     if sys.argv[1:] == ["adapted"]:
         return adapted()
+    if sys.argv[1:] == ["bc"]:
+        return bc_cases()
     if sys.argv[1:] == ["digests"]:
         committed_output_digests()
         return fixture_output_digests()
@@ -280,6 +304,7 @@ def main():
                    mesh_src=os.path.join(REF, "wedge40_120.msh"))
     three_d()
     adapted()
+    bc_cases()
 
 
 if __name__ == "__main__":
