@@ -328,9 +328,16 @@ def main():
     per_iter = {k: cnt * ms / a.timing_iters for k, (cnt, ms) in stats.items()}
     dom = max(per_iter, key=per_iter.get)
     kind = {"flux": "flux", "update": "update", "grad_gg": "grad", "grad_lsq": "grad", "bc": "bc",
-            "stage": "stage", "stage_last": "stage"}.get(dom, dom)
+            "stage": "stage", "stage_last": "stage", "update_levels_block": "update",
+            "update_level": "update"}.get(dom, dom)
     nbytes = alg_bytes(kind, N, G, F, nnz, dim, b, last_stage=(dom == "stage_last"))
     mean_ms = stats[dom][1]
+    note = None
+    if dom in ("update_levels_block", "update_level"):
+        # time step mode 1 in the reference's order: each stage's update is one sweep over the cells'
+        # dependency levels (one launch, or one per level); rate over a whole stage's sweep
+        mean_ms = stats[dom][0] * stats[dom][1] / (a.timing_iters * 4)
+        note = "reference-ordered update sweep (a chain of dependency levels): latency-bound, not HBM-bound"
     achieved = nbytes / (mean_ms * 1e-3) / 1e9
     # HBM traffic of the dominant kernel: rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs (each scaled by
     # the counters' reading for a copy kernel of known size), recorded under profiles/ for the default
@@ -346,6 +353,8 @@ def main():
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "alg_bytes_per_launch": int(nbytes), "mean_ms": round(mean_ms, 4)}
+    if note:
+        roofline["note"] = note
     whole = survey_bytes_per_cell_iteration(dim, b, N, F)
     out = {
         "metric": METRIC, "value": round(cell_its, 1), "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,

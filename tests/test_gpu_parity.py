@@ -211,3 +211,28 @@ def test_full_size_wedge_one_iteration_matches_oracle(tmp_path):
             res = s.run(1)
             assert np.array_equal(res, ref_res), fused
             assert _first_mismatch(s.state(), ref) is None, (fused, _first_mismatch(s.state(), ref))
+
+
+@pytest.mark.parametrize("numbering", [1, 0])
+def test_reference_ordered_sweep_on_a_large_mesh(tmp_path, numbering):
+    """Time step mode 1 in the reference's order on a mesh whose levels are
+    wider than one workgroup's sweep (800 x 400 = 320,000 cells): the
+    cooperative sweep (one launch per stage, a grid barrier per level) is
+    bit-identical to the oracle over three iterations."""
+    p = ss.wedge_params(nx=800, ny=400, jitter=0.1, jitter_seed=4, numbering=numbering)
+    prefix = str(tmp_path / "c")
+    ss.write_case_files(prefix, time_mode=1, iterations=3, flux=3, order=2, grad=1, limiter=2, cfl=0.8)
+    case = ss.Case.wedge(p, prefix)
+    raw = case.raw()
+    raw["dim"] = 2
+    o = Oracle(None, prefix, raw=raw)
+    ref_res = o.run(3)
+    ref = o.state()
+    with ss.Solver(case) as s:
+        s.kernel_timing(True)
+        res = s.run(3)
+        launches = s.kernel_stats("update_levels_grid")[0]
+        print("numbering %d: %d cooperative sweeps" % (numbering, launches), flush=True)
+        assert launches == 3 * 4
+        assert np.array_equal(res, ref_res)
+        assert _first_mismatch(s.state(), ref) is None, _first_mismatch(s.state(), ref)
