@@ -149,7 +149,7 @@ def run_warmstart(a):
 def run_sweep(a):
     """Batched sweep: `instances` wedges (cells log-uniform, Mach and ramp
     angle uniform, seeded), all solved to a `tol` residual drop in one batch
-    (persistent work-queue launches, hipGraph replay, autotuned tiles)."""
+    (one workgroup per active tile, hipGraph replay, autotuned tiles)."""
     import numpy as np
 
     import supersonicflowsolver_amd as ss
