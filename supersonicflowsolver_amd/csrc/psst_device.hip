@@ -756,30 +756,153 @@ __global__ void __launch_bounds__(1024) k_update_levels_block(Geom g, State<T> s
     }
 }
 
-// Same, one cooperative launch over several workgroups: the cells of a level
-// are spread over the grid and a grid-wide barrier separates the levels.
-// Every workgroup runs the same number of barriers; the cooperative launch
-// guarantees that all of them are resident.
+// Tables of the ordered sweep on large meshes, by position in the level
+// order (so a level's loads are contiguous): the cell, its volume, and its
+// face list as fixed-width entries (neighbour device id, with bit 30 set
+// when the neighbour comes earlier in the reference's update loop; -1 past
+// the end), face normals and areas.  R holds each cell's residual.
+struct OrderedTables {
+    const int *__restrict__ cell;   // [N]
+    const double *__restrict__ vol; // [N]
+    const int *__restrict__ nb;     // [W][N]
+    const double *__restrict__ fn;  // [DIM][W][N]
+    const double *__restrict__ fa;  // [W][N]
+    void *R;                        // [DIM+2][N], state precision
+    int W;                          // at most kOrderedW
+};
+constexpr int kOrderedW = 8;
+constexpr int kEarlier = 1 << 30;
+
+// Pre-pass of the ordered sweep: every cell's residual (the ordered gather
+// of update_cell) at its level-order position, and the residual max-norm;
+// neither depends on the sweep.
 template <int DIM, typename T>
-__global__ void __launch_bounds__(kBlock) k_update_levels_grid(Geom g, State<T> s, Phys ph, T alpha, int first_stage,
-                                                               int last_stage, unsigned long long *res_row,
-                                                               const int *__restrict__ cells,
-                                                               const int *__restrict__ level_off, int n_levels) {
-    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
-    unsigned long long rmax[DIM + 2];
+__global__ void __launch_bounds__(kBlock) k_ordered_residual(Geom g, State<T> s, OrderedTables ot,
+                                                             unsigned long long *res_row) {
+    constexpr int WS = DIM + 2;
+    const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    unsigned long long rmax[WS];
 #pragma unroll
-    for (int j = 0; j < DIM + 2; ++j) rmax[j] = 0;
+    for (int j = 0; j < WS; ++j) rmax[j] = 0;
+    if (p < g.N) {
+        const int c = ot.cell[p];
+        T R[WS];
+#pragma unroll
+        for (int j = 0; j < WS; ++j) R[j] = T(0);
+        for (int i = g.voff[c]; i < g.voff[c + 1]; ++i) {
+            const int f = g.vface[i];
+            const bool first = (g.f0[f] == c);
+#pragma unroll
+            for (int j = 0; j < WS; ++j) {
+                T v = s.FA[(size_t)j * g.F + f];
+                R[j] = first ? R[j] - v : R[j] + v;
+            }
+        }
+        T *Rp = static_cast<T *>(ot.R);
+#pragma unroll
+        for (int j = 0; j < WS; ++j) {
+            Rp[(size_t)j * g.N + p] = R[j];
+            rmax[j] = abs_bits(R[j]);
+        }
+    }
+    residual_max<WS>(rmax, res_row);
+}
+
+// One cell of the ordered sweep from the tables: update_cell with
+// dt_local1 (psstLocalAccTime1::calculate_at, psstLocalTimeCalculation.cpp:
+// 90-131) in the same operation order.  Everything but the earlier
+// neighbours' new state is loaded straight from the position.
+template <int DIM, typename T>
+__device__ __forceinline__ void ordered_update(const Geom &g, const State<T> &s, const Phys &ph, T alpha,
+                                               int last_stage, const OrderedTables &ot, int p) {
+    constexpr int WS = DIM + 2;
+    const int c = ot.cell[p];
+    const T *Rp = static_cast<const T *>(ot.R);
+    T R[WS], w0[WS], uc[DIM];
+#pragma unroll
+    for (int j = 0; j < WS; ++j) {
+        R[j] = Rp[(size_t)j * g.N + p];
+        w0[j] = s.W0[(size_t)j * g.N + c];
+    }
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) uc[k] = s.q[(size_t)k * g.NT + c];
+    const T Tc = s.q[(size_t)(DIM + 1) * g.NT + c];
+    const T V = T(ot.vol[p]);
+    int nbw[kOrderedW];
+    double fnw[kOrderedW][DIM], faw[kOrderedW];
+#pragma unroll
+    for (int w = 0; w < kOrderedW; ++w) {
+        const bool in = w < ot.W;
+        nbw[w] = in ? ot.nb[(size_t)w * g.N + p] : -1;
+        faw[w] = in ? ot.fa[(size_t)w * g.N + p] : 0.0;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) fnw[w][k] = in ? ot.fn[((size_t)k * ot.W + w) * g.N + p] : 0.0;
+    }
+    // neighbour states: the new one for neighbours earlier in the update
+    // loop, the stage-start one otherwise (ghosts included)
+    T un[kOrderedW][DIM], Tn[kOrderedW];
+#pragma unroll
+    for (int w = 0; w < kOrderedW; ++w) {
+        const int e = nbw[w] < 0 ? 0 : nbw[w];
+        const int nb = e & (kEarlier - 1);
+        const T *src = (e & kEarlier) ? s.qn : s.q;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) un[w][k] = nbw[w] < 0 ? T(0) : src[(size_t)k * g.NT + nb];
+        Tn[w] = nbw[w] < 0 ? T(0) : src[(size_t)(DIM + 1) * g.NT + nb];
+    }
+    const T gr = T(ph.gamma) * T(ph.rm);
+    T cc = dsqrt(gr * Tc);
+    T den = T(0), nn = T(0);
+#pragma unroll
+    for (int w = 0; w < kOrderedW; ++w) {
+        if (nbw[w] < 0) break;
+        T nv = T(0);
+        T cn = (dsqrt(gr * Tn[w]) + cc) * T(0.5);
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            T t = T(fnw[w][k]);
+            nv += uc[k] * t;
+            nn += un[w][k] * t;
+        }
+        nn = (nn + nv) * T(0.5);
+        den += (fabs(nn) + cn) * T(faw[w]);
+    }
+    const T dt = fdiv(T(ph.cfl) * V, den);
+    T wn[WS];
+#pragma unroll
+    for (int j = 0; j < WS; ++j) wn[j] = w0[j] - fdiv(dt * alpha * R[j], V);
+    if (last_stage)
+#pragma unroll
+        for (int j = 0; j < WS; ++j) s.W0[(size_t)j * g.N + c] = wn[j];
+    const T rho = wn[DIM];
+    T vsq = T(0);
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+        T u = fdiv(wn[k], rho);
+        s.qn[(size_t)k * g.NT + c] = u;
+        vsq += u * u;
+    }
+    T Tnew = fdiv(fdiv(wn[DIM + 1], rho) - (vsq * T(0.5)), T(ph.cv));
+    s.qn[(size_t)DIM * g.NT + c] = rho * T(ph.rm) * Tnew;
+    s.qn[(size_t)(DIM + 1) * g.NT + c] = Tnew;
+    s.qn[(size_t)(DIM + 2) * g.NT + c] = rho;
+}
+
+// The ordered sweep as one cooperative launch over several workgroups: the
+// positions of a level are spread over the grid and a grid-wide barrier
+// separates the levels.  Every workgroup runs the same number of barriers;
+// the cooperative launch guarantees that all of them are resident.
+template <int DIM, typename T>
+__global__ void __launch_bounds__(kBlock) k_update_levels_grid(Geom g, State<T> s, Phys ph, T alpha, int last_stage,
+                                                               OrderedTables ot, const int *__restrict__ level_off,
+                                                               int n_levels) {
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
     const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x), nthr = (int)(gridDim.x * blockDim.x);
     for (int l = 0; l < n_levels; ++l) {
         const int b = level_off[l], e = level_off[l + 1];
-        for (int i = b + tid; i < e; i += nthr) {
-            const int c = cells[i];
-            update_cell<DIM, T, PSST_DT_LOCAL1>(g, s, ph, c, alpha, first_stage, last_stage, rmax,
-                                                NbOrdered<DIM, T>{s.q, s.qn, g.rank, g.NT, c});
-        }
+        for (int p = b + tid; p < e; p += nthr) ordered_update<DIM, T>(g, s, ph, alpha, last_stage, ot, p);
         grid.sync();
     }
-    residual_max<DIM + 2>(rmax, res_row);
 }
 
 // ---------------------------------------------------------------------------
@@ -1462,6 +1585,7 @@ struct psst_ctx {
     std::vector<int> level_off;
     int max_level = 0;   // cells in the widest level
     int coop_blocks = -1;  // workgroups of the cooperative sweep (-1: not sized yet, 0: unavailable)
+    OrderedTables ot{};    // tables of the cooperative sweep (ot.cell == nullptr: not built)
     // residual normalisation for the tolerance test
     std::vector<double> res0;
     char *snap = nullptr;  // state at the start of a chunk of a run with a tolerance
@@ -1809,11 +1933,12 @@ void run_update(psst_ctx *c, int k, unsigned long long *res_row) {
         if (c->max_level <= 256 || c->N <= 65536) {
             launch(c, "update_levels_block", k_update_levels_block<DIM, T>, dim3(1), dim3(1024), c->geom, st, c->ph,
                    alpha, first, last, res_row, (const int *)c->level_cells, (const int *)c->level_off_dev, nl);
-        } else if (!c->batch && coop_sweep_blocks<DIM, T>(c) > 0) {
+        } else if (!c->batch && c->ot.cell && coop_sweep_blocks<DIM, T>(c) > 0) {
             // (a batch captures its iterations into graphs: it keeps the per-level launches)
+            launch(c, "ordered_residual", k_ordered_residual<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st,
+                   c->ot, res_row);
             launch_coop(c, "update_levels_grid", k_update_levels_grid<DIM, T>, dim3(c->coop_blocks), dim3(kBlock),
-                        c->geom, st, c->ph, alpha, first, last, res_row, (const int *)c->level_cells,
-                        (const int *)c->level_off_dev, nl);
+                        c->geom, st, c->ph, alpha, last, c->ot, (const int *)c->level_off_dev, nl);
         } else {
             for (int l = 0; l < nl; ++l) {
                 int b = c->level_off[l], n = c->level_off[l + 1] - b;
@@ -2171,6 +2296,32 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
             c->max_level = widest;
             c->level_cells = dev_upload(c.get(), cells.data(), cells.size());
             c->level_off_dev = dev_upload(c.get(), c->level_off.data(), c->level_off.size());
+            // tables of the cooperative sweep (large meshes with wide levels)
+            int W = 0;
+            for (int i = 0; i < N; ++i) W = std::max(W, m->list_offsets[i + 1] - m->list_offsets[i]);
+            if (N > 65536 && widest > 256 && W <= kOrderedW) {
+                std::vector<int> onb((size_t)W * N, -1);
+                std::vector<double> ofn((size_t)dim * W * N, 0.0), ofa((size_t)W * N, 0.0), ovol((size_t)N);
+                for (int p = 0; p < N; ++p) {
+                    const int i = d2c[(size_t)cells[(size_t)p]];
+                    ovol[(size_t)p] = m->cell_volume[i];
+                    int w = 0;
+                    for (int k = m->list_offsets[i]; k < m->list_offsets[i + 1]; ++k, ++w) {
+                        const int f = m->list_faces[k], nbc = lnb_c[(size_t)k];
+                        onb[(size_t)w * N + p] = c2d[(size_t)nbc] | (nbc < i ? kEarlier : 0);
+                        ofa[(size_t)w * N + p] = m->face_area[f];
+                        for (int x = 0; x < dim; ++x)
+                            ofn[((size_t)x * W + w) * N + p] = m->face_normal[(size_t)x * F + f];
+                    }
+                }
+                c->ot.cell = c->level_cells;
+                c->ot.vol = dev_upload(c.get(), ovol.data(), ovol.size());
+                c->ot.nb = dev_upload(c.get(), onb.data(), onb.size());
+                c->ot.fn = dev_upload(c.get(), ofn.data(), ofn.size());
+                c->ot.fa = dev_upload(c.get(), ofa.data(), ofa.size());
+                c->ot.R = dev_alloc<double>(c.get(), (size_t)(dim + 2) * N);
+                c->ot.W = W;
+            }
         }
         // ---- device geometry ----
         // Device face order: faces sorted by their lower device cell id, so
@@ -2344,7 +2495,8 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                            "update_level", "update_levels_block",
                            "update_levels_grid", "bc",
                            "stage",        "grad_gg_tile",
-                           "grad_lsq_tile", "batch_check"};
+                           "grad_lsq_tile", "batch_check",
+                           "ordered_residual"};
     } catch (const std::exception &e) {
         g_create_error = e.what();
         for (void *p : c->allocs) (void)hipFree(p);
