@@ -1765,8 +1765,6 @@ int batch_grid_tiles(int count, int n_tiles) {
     return std::min(cap, n_tiles);
 }
 
-// Work list of a launch (list == nullptr: unbatched).
-BatchDev launch_bd(psst_ctx *c, int) { return c->bd; }
 
 size_t stage_shm(const psst_ctx *c, size_t sb) {
     const int NQ = c->dim + 3, NV = c->dim + 2;
@@ -1778,7 +1776,7 @@ template <int DIM, typename T, int NTHR>
 void run_grad_tile(psst_ctx *c) {
     State<T> st = state<T>(c);
     const size_t shm = grad_shm(c, sizeof(T));
-    const BatchDev bd = launch_bd(c, 0);
+    const BatchDev bd = c->bd;  // work list (list == nullptr: unbatched)
     if (c->s.grad == PSST_GRAD_GREEN_GAUSS) {
         auto k = k_grad_gg_tile<DIM, T, NTHR>;
         launch_shm(c, "grad_gg_tile", k, tile_grid(c, k, NTHR, shm), dim3(NTHR), shm, c->geom, c->td, st, bd);
@@ -1810,7 +1808,7 @@ void run_stage_k(psst_ctx *c, int k, unsigned long long *res_row) {
     const size_t shm = stage_shm(c, sizeof(T));
     auto kern = k_stage<DIM, T, FLUX, ORDER, LIM, LC, NTHR>;
     launch_shm(c, "stage", kern, tile_grid(c, kern, NTHR, shm), dim3(NTHR), shm, c->geom, c->td, st, c->ph, alpha,
-               last, res_row, (const BCDev *)c->bc, launch_bd(c, 1 + k));
+               last, res_row, (const BCDev *)c->bc, c->bd);
 }
 
 template <int DIM, typename T, int FLUX, int ORDER, int LIM, int NTHR>
