@@ -5,7 +5,7 @@
  * starts from the .par freestream or a .sav restart
  * (psstFiniteVolumeGrid.cpp:641-702, init mode 0/1); the network's output is
  * handed to the solver as such an initial state (psst_set_state), and the
- * solver still runs to its tolerance, so it remains the judge of the
+ * solver still runs to its tolerance, so it remains the arbiter of the
  * surrogate's error.
  *
  * Network: n_in inputs, `depth` hidden layers of `width` units with SiLU,

@@ -3,7 +3,7 @@
 A coordinate MLP maps (x, y, Mach, ramp angle) to (rho, u, v, p).  It is
 trained on converged fields that the batched solver produces, and a new
 solve starts from its prediction instead of the freestream.  The plain
-solver still runs to the same residual tolerance, so it stays the judge of
+solver still runs to the same residual tolerance, so it stays the arbiter of
 the surrogate's error; the residual drop is normalised by the freestream
 start's first residual, so the warm start cannot lower its own denominator.
 """

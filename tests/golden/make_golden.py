@@ -1,5 +1,5 @@
 """Writes the golden fixtures under tests/golden/ (run in the container that
-holds the reference tree; the GPU box only reads the committed results).
+holds the reference tree; GPU runs only read the committed results).
 
 Fixture kinds:
   ref_wedge80x240/   the reference's OWN committed run (wedge80_240.msh with
