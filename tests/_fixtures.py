@@ -14,6 +14,8 @@ MATRIX_CASES = sorted(d for d in (os.listdir(GOLDEN) if os.path.isdir(GOLDEN) el
 THREE_D_CASES = sorted(d for d in (os.listdir(GOLDEN) if os.path.isdir(GOLDEN) else []) if d.startswith("r3d_"))
 # subsonic outlet (25) and isothermal walls (11)
 BC_CASES = sorted(d for d in (os.listdir(GOLDEN) if os.path.isdir(GOLDEN) else []) if d.startswith("b_"))
+# restart from the reference's own .sav (.inp init mode 1)
+RESTART_CASES = sorted(d for d in (os.listdir(GOLDEN) if os.path.isdir(GOLDEN) else []) if d.startswith("restart_"))
 # adapted (hanging-node) meshes of the reference: parent zones, face/cell trees
 ADAPTED_CASES = sorted(d for d in (os.listdir(GOLDEN) if os.path.isdir(GOLDEN) else []) if d.startswith("a_"))
 
@@ -33,7 +35,7 @@ def unpack(name, workdir):
         elif fn.endswith(".xz"):
             with lzma.open(p) as f, open(os.path.join(workdir, fn[:-3]), "wb") as g:
                 g.write(f.read())
-        elif fn.split(".")[-1] in ("inp", "bvp", "btq", "par", "prop"):
+        elif fn.split(".")[-1] in ("inp", "bvp", "btq", "par", "prop", "sav"):
             shutil.copy(p, workdir)
             prefix = os.path.join(workdir, fn.rsplit(".", 1)[0])
     exp = np.load(os.path.join(src, "expected.npz"), allow_pickle=False)
@@ -46,9 +48,10 @@ def unpack(name, workdir):
     return mesh, prefix, {"prim": exp["prim"], "residual": [str(x) for x in exp["residual"]]}, meta
 
 
-def residual_lines(res):
-    """Residual rows formatted like the reference's residual file (time column dropped)."""
+def residual_lines(res, first=1):
+    """Residual rows formatted like the reference's residual file (time column
+    dropped), numbered from iteration `first`."""
     out = []
     for i, row in enumerate(res):
-        out.append("%d  " % (i + 1) + " ".join("%.3e" % v for v in row))
+        out.append("%d  " % (i + first) + " ".join("%.3e" % v for v in row))
     return out

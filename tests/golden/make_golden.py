@@ -16,6 +16,7 @@ Fixture kinds:
                      the reference's quad2D.msh, 12 iterations each.
   r3d_*              3D: the reference's wedge3D.msh and quad3D.msh hex meshes
                      (stored once under _meshes/), Green-Gauss schemes.
+  restart_*          a restart from the reference's own .sav (init mode 1).
   b_*                the subsonic outlet (25) and isothermal walls (11) on the
                      10x30 wedge, three schemes.
   a_*                adapted (hanging-node) meshes: the reference's wedge3.msh,
@@ -274,6 +275,37 @@ def bc_cases():
                        12)
 
 
+def restart_case():
+    """Restart from a checkpoint (.inp init mode 1, private_load_parameter,
+    psstFiniteVolumeGrid.cpp:559-604): the reference runs 12 iterations and
+    saves; a second run with init mode 1 reads that .sav (iteration 13, fields
+    at 11 decimals) and runs its .inp's 24 iterations, 13 to 36 (the
+    iteration count is relative to the restart).  Stored: the .sav it read,
+    the settings of the second run, its final state and the residual file of
+    both runs (lines 1-12, then 13-36 appended)."""
+    import supersonicflowsolver_amd as ss
+    name = "restart_hllc_o2_gg_mm_lc2_rk4"
+    d = os.path.join(HERE, name)
+    os.makedirs(d, exist_ok=True)
+    kw = dict(cfl=0.6, rk_levels=4, flux=3, order=2, grad=1, limiter=2, time_mode=2, mach=2.5, save_step=100000)
+    with tempfile.TemporaryDirectory() as w:
+        mesh_path = os.path.join(w, "mesh.msh")
+        ss.write_wedge_msh(ss.wedge_params(**SMALL), mesh_path)
+        with open(mesh_path, "rb") as f:
+            write_xz(os.path.join(d, "mesh.msh.xz"), f.read())
+        ss.write_case_files(os.path.join(w, "case"), iterations=12, **kw)
+        run_reference(w, "mesh.msh", "case")
+        shutil.copy(os.path.join(w, "case.sav"), os.path.join(d, "case.sav"))
+        ss.write_case_files(os.path.join(w, "case"), iterations=24, init_mode=1, **kw)
+        vals, res = run_reference(w, "mesh.msh", "case")
+        for ext in ("inp", "bvp", "btq", "par", "prop"):
+            shutil.copy(os.path.join(w, "case." + ext), d)
+        digests = output_digests(w, "case")
+    save_expected(d, vals, 2, res, {"iterations": 24, "first_iteration": 13, "wedge_params": SMALL, "case": kw,
+                                    "mesh_source": "supersonicflowsolver_amd wedge generator",
+                                    "reference_output_sha256": digests})
+
+
 def adapted():
     for name, mesh, flux, order, grad, lim, lc, rk, cfl in ADAPTED:
         kw = dict(ADAPTED_CASE, cfl=cfl, rk_levels=rk, flux=flux, order=order, grad=grad, limiter=lim,
@@ -290,6 +322,8 @@ def main():
         return adapted()
     if sys.argv[1:] == ["bc"]:
         return bc_cases()
+    if sys.argv[1:] == ["restart"]:
+        return restart_case()
     if sys.argv[1:] == ["digests"]:
         committed_output_digests()
         return fixture_output_digests()
@@ -305,6 +339,7 @@ def main():
     three_d()
     adapted()
     bc_cases()
+    restart_case()
 
 
 if __name__ == "__main__":

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import supersonicflowsolver_amd as ss
-from _fixtures import ADAPTED_CASES, BC_CASES, GENERATED_CASES, MATRIX_CASES, THREE_D_CASES, residual_lines, unpack
+from _fixtures import ADAPTED_CASES, BC_CASES, GENERATED_CASES, RESTART_CASES, MATRIX_CASES, THREE_D_CASES, residual_lines, unpack
 from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -272,3 +272,23 @@ def test_reference_ordered_sweep_3d_and_fp32(tmp_path):
             scale[2] = scale[0]
             print("FP32 relative L2 per field", err / scale, flush=True)
             assert (err / scale).max() < 1e-5
+
+
+@pytest.mark.parametrize("name", RESTART_CASES)
+def test_restart_from_the_reference_sav(tmp_path, name):
+    """.inp init mode 1 on the device: the run from the reference's own .sav is
+    bit-identical to the reference build's second run; psst_case_solve appends
+    the residual rows from the saved iteration on and writes the same .dat."""
+    mesh, prefix, exp, meta = unpack(name, str(tmp_path))
+    case = ss.Case.open(mesh, prefix)
+    first = meta["first_iteration"]
+    assert (case.first_iter, case.last_iter) == (first, first + meta["iterations"] - 1)
+    with ss.Solver(case) as s:
+        res = s.run(meta["iterations"])
+        assert _first_mismatch(s.state(), exp["prim"]) is None, _first_mismatch(s.state(), exp["prim"])
+    assert residual_lines(res, first) == exp["residual"][first - 1:]
+    case.solve()
+    with open(prefix + "_residual.dat") as f:
+        rows = [" ".join([l.split(" ")[0]] + l.rstrip("\n").split(" ")[2:]) for l in f]
+    assert rows == exp["residual"][first - 1:]
+    assert _sha(prefix + ".dat") == meta["reference_output_sha256"]["dat"]

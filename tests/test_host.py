@@ -28,7 +28,8 @@ def test_library_exports_every_declared_symbol():
 
 
 @pytest.mark.parametrize("name", ["ref_wedge80x240", "c1_lc1", "m_hll_o2_lsq_mm_lc2_rk3", "m_ausm_o2_lsq_vl_lc0_rk2",
-                                  "a_wedge3_ausm_o2_lsq_va_lc0_rk3", "a_wedge5_hllc_o2_gg_va_lc0_rk2"])
+                                  "a_wedge3_ausm_o2_lsq_va_lc0_rk3", "a_wedge5_hllc_o2_gg_va_lc0_rk2",
+                                  "restart_hllc_o2_gg_mm_lc2_rk4"])
 def test_host_setup_matches_oracle(tmp_path, name):
     mesh, prefix, _, _ = unpack(name, str(tmp_path))
     case = ss.Case.open(mesh, prefix)

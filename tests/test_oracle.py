@@ -19,7 +19,7 @@ import os
 import numpy as np
 import pytest
 
-from _fixtures import ADAPTED_CASES, BC_CASES, GENERATED_CASES, MATRIX_CASES, THREE_D_CASES, residual_lines, unpack
+from _fixtures import ADAPTED_CASES, BC_CASES, GENERATED_CASES, RESTART_CASES, MATRIX_CASES, THREE_D_CASES, residual_lines, unpack
 from oracle.oracle import Oracle
 
 
@@ -63,6 +63,20 @@ def test_reference_build_dumps(tmp_path, name):
     bad = np.argwhere(st != exp["prim"])
     assert bad.size == 0, "first mismatch at %s: %r vs %r" % (bad[0], st[tuple(bad[0])], exp["prim"][tuple(bad[0])])
     assert residual_lines(res) == exp["residual"]
+
+
+@pytest.mark.parametrize("name", RESTART_CASES)
+def test_restart_from_the_reference_sav(tmp_path, name):
+    """.inp init mode 1: the state read back from the reference's own .sav
+    (11 decimals), then the .inp's iteration count from the saved iteration
+    on: final state bit-identical to the reference build's, residual rows
+    equal to the lines its second run appended."""
+    mesh, prefix, exp, meta = unpack(name, str(tmp_path))
+    o = Oracle(mesh, prefix)
+    res = o.run(meta["iterations"])
+    assert np.array_equal(o.state(), exp["prim"])
+    first = meta["first_iteration"]
+    assert residual_lines(res, first) == exp["residual"][first - 1:]
 
 
 def test_matrix_covers_every_scheme():
