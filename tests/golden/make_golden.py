@@ -268,6 +268,13 @@ BC_CASES = [
 ]
 
 
+def custom_rk_case():
+    """Custom Runge-Kutta coefficients (.inp line 7 = 1, coefficients on line 8)."""
+    generated_case("m_hllc_o2_gg_mm_lc2_rkc3", SMALL,
+                   dict(cfl=0.6, rk_levels=3, rk_custom=[0.2, 0.45, 1.0], flux=3, order=2, grad=1, limiter=2,
+                        time_mode=2, mach=2.5, save_step=100000), 12)
+
+
 def bc_cases():
     for name, flux, order, grad, lim, lc, rk in BC_CASES:
         generated_case(name, SMALL, dict(cfl=0.6, rk_levels=rk, flux=flux, order=order, grad=grad, limiter=lim,
@@ -324,6 +331,8 @@ def main():
         return bc_cases()
     if sys.argv[1:] == ["restart"]:
         return restart_case()
+    if sys.argv[1:] == ["customrk"]:
+        return custom_rk_case()
     if sys.argv[1:] == ["digests"]:
         committed_output_digests()
         return fixture_output_digests()
@@ -340,6 +349,7 @@ def main():
     adapted()
     bc_cases()
     restart_case()
+    custom_rk_case()
 
 
 if __name__ == "__main__":

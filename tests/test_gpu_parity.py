@@ -292,3 +292,26 @@ def test_restart_from_the_reference_sav(tmp_path, name):
         rows = [" ".join([l.split(" ")[0]] + l.rstrip("\n").split(" ")[2:]) for l in f]
     assert rows == exp["residual"][first - 1:]
     assert _sha(prefix + ".dat") == meta["reference_output_sha256"]["dat"]
+
+
+def test_case_solve_reproduces_the_reference_run(tmp_path):
+    """The reference's main() on its own case (wedge80_240.msh, wedge.inp: 3000
+    iterations, time step mode 1 in the reference's order, a save every 1000)
+    run by psst_case_solve: its residual file, .sav, .dat and .cas equal the
+    ones the reference committed."""
+    mesh, prefix, exp, meta = unpack("ref_wedge80x240", str(tmp_path))
+    with open(prefix + ".sav") as f:
+        sav = f.read()
+    with open(prefix + "_residual.dat") as f:
+        ref_rows = [" ".join([l.split(" ")[0]] + l.rstrip("\n").split(" ")[2:]) for l in f]
+    for ext in (".sav", "_residual.dat"):
+        os.remove(prefix + ext)
+    case = ss.Case.open(mesh, prefix)
+    case.solve()
+    with open(prefix + "_residual.dat") as f:
+        rows = [" ".join([l.split(" ")[0]] + l.rstrip("\n").split(" ")[2:]) for l in f]
+    assert len(rows) == 3000 and rows == ref_rows
+    with open(prefix + ".sav") as f:
+        assert f.read() == sav
+    assert _sha(prefix + ".dat") == meta["committed_output_sha256"]["dat"]
+    assert _sha(prefix + ".cas") == meta["committed_output_sha256"]["cas"]
