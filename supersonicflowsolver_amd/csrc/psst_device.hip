@@ -1551,6 +1551,11 @@ struct BatchState {
     unsigned long long *res = nullptr;
     double *res0 = nullptr, *hist = nullptr, *drop = nullptr;
     int *active = nullptr, *conv = nullptr, *frozen_buf = nullptr, *iter = nullptr;
+    // pinned host copies read back at every chunk (active flags then the
+    // active-tile count; the residual ring), so the copies go straight to
+    // host memory
+    int *h_flags = nullptr;
+    double *h_ring = nullptr;
     int64_t iter_host = 0;
     bool w_dirty = false;
     bool use_graph = true;
@@ -2812,10 +2817,13 @@ psst_status psst_copy_calibration(int64_t bytes, int32_t reps, double *mean_ms) 
 void psst_ctx_destroy(psst_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->batch)
+    if (c->batch) {
         for (auto &kv : c->batch->exec)
             for (hipGraphExec_t &e : kv.second)
                 if (e) (void)hipGraphExecDestroy(e);
+        if (c->batch->h_flags) (void)hipHostFree(c->batch->h_flags);
+        if (c->batch->h_ring) (void)hipHostFree(c->batch->h_ring);
+    }
     for (auto &p : c->pending) {
         (void)hipEventDestroy(p.second.first);
         (void)hipEventDestroy(p.second.second);
@@ -3210,6 +3218,13 @@ psst_status psst_batch_create(int32_t n, const psst_mesh_desc *meshes, const pss
         bs->drop = dev_upload(cx, zd.data(), zd.size());
         bs->hist = dev_alloc<double>(cx, (size_t)bs->H * n * ws);
         c->batch = std::move(bs);
+        // after the move, so that the context's destroy frees them on any
+        // later failure
+        BatchState &b = *c->batch;
+        hip_check(hipHostMalloc(reinterpret_cast<void **>(&b.h_flags), sizeof(int) * ((size_t)n + 1)),
+                  "hipHostMalloc");
+        hip_check(hipHostMalloc(reinterpret_cast<void **>(&b.h_ring), sizeof(double) * (size_t)b.H * n * ws),
+                  "hipHostMalloc");
         *out = bt.release();
         return PSST_OK;
     } catch (const std::exception &e) {
@@ -3340,9 +3355,10 @@ psst_status psst_batch_run(psst_batch *bt, int32_t max_iter, double tol, int32_t
         hip_check(hipMemcpy(b.active, active.data(), sizeof(int) * n, hipMemcpyHostToDevice), "hipMemcpy");
         hip_check(hipMemcpy(b.conv, conv.data(), sizeof(int) * n, hipMemcpyHostToDevice), "hipMemcpy");
         launch_check(c, tol, c->cur, 1);
-        int count = 0;
-        hip_check(hipMemcpyAsync(&count, b.count, sizeof(int), hipMemcpyDeviceToHost, c->stream), "hipMemcpy");
+        hip_check(hipMemcpyAsync(b.h_flags + n, b.count, sizeof(int), hipMemcpyDeviceToHost, c->stream),
+                  "hipMemcpy");
         hip_check(hipStreamSynchronize(c->stream), "psst_batch_run");
+        int count = b.h_flags[n];
         // the active list only shrinks during a run, so a capacity chosen at
         // a chunk's start holds it for the whole chunk
         b.grid_tiles = batch_grid_tiles(count, c->n_tiles);
@@ -3350,7 +3366,6 @@ psst_status psst_batch_run(psst_batch *bt, int32_t max_iter, double tol, int32_t
         const int64_t it0 = b.iter_host;
         int done = 0;
         bool any = true, first = true;
-        std::vector<double> ring;
         while (done < max_iter && any) {
             const int m = std::min(b.H, max_iter - done);
             for (int k = 0; k < m; ++k) {
@@ -3358,22 +3373,23 @@ psst_status psst_batch_run(psst_batch *bt, int32_t max_iter, double tol, int32_t
                 else batch_graph_iteration(c, tol);
                 first = false;
             }
-            if (res_hist) {
-                ring.resize((size_t)b.H * n * ws);
-                hip_check(hipMemcpyAsync(ring.data(), b.hist, ring.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                         c->stream),
+            if (res_hist)
+                hip_check(hipMemcpyAsync(b.h_ring, b.hist, sizeof(double) * (size_t)b.H * n * ws,
+                                         hipMemcpyDeviceToHost, c->stream),
                           "hipMemcpy");
-            }
-            hip_check(hipMemcpyAsync(active.data(), b.active, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream),
+            hip_check(hipMemcpyAsync(b.h_flags, b.active, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream),
                       "hipMemcpy");
-            hip_check(hipMemcpyAsync(&count, b.count, sizeof(int), hipMemcpyDeviceToHost, c->stream), "hipMemcpy");
+            hip_check(hipMemcpyAsync(b.h_flags + n, b.count, sizeof(int), hipMemcpyDeviceToHost, c->stream),
+                      "hipMemcpy");
             hip_check(hipStreamSynchronize(c->stream), "psst_batch_run");
+            std::copy(b.h_flags, b.h_flags + n, active.begin());
+            count = b.h_flags[n];
             b.grid_tiles = batch_grid_tiles(count, c->n_tiles);
             if (c->timing) flush_timing(c);
             if (res_hist)
                 for (int k = 0; k < m; ++k) {
                     const size_t row = (size_t)((b.iter_host + k) % b.H);
-                    std::memcpy(res_hist + (size_t)(done + k) * n * ws, ring.data() + row * n * ws,
+                    std::memcpy(res_hist + (size_t)(done + k) * n * ws, b.h_ring + row * n * ws,
                                 sizeof(double) * n * ws);
                 }
             b.iter_host += m;
