@@ -1196,7 +1196,7 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, con
         // 3% (FP64) and 6% (FP32) faster than loading each where it is used.
         T w0v[WS];
 #pragma unroll
-        for (int j = 0; j < WS; ++j) w0v[j] = s.W0[(size_t)j * g.N + c];
+        for (int j = 0; j < WS; ++j) w0v[j] = __builtin_nontemporal_load(&s.W0[(size_t)j * g.N + c]);
         const T Vh = T(g.vol[c]);
         T Sh[DIM];
 #pragma unroll
@@ -1274,21 +1274,26 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, con
             unsigned long long ab = abs_bits(R[j]);
             rmax[j] = rmax[j] > ab ? rmax[j] : ab;
         }
+        // the new state and the iteration-start snapshot are written once
+        // and not read again in this launch (the ghost pass reads only the
+        // tile's own interior neighbours): non-temporal stores and loads,
+        // measured 0.6% (FP64) and 1.2% (FP32) faster per iteration on the
+        // 4.9M-cell wedge; the same hint on the face geometry loads was slower
         if (last_stage)
 #pragma unroll
-            for (int j = 0; j < WS; ++j) s.W0[(size_t)j * g.N + c] = w[j];
+            for (int j = 0; j < WS; ++j) __builtin_nontemporal_store(w[j], &s.W0[(size_t)j * g.N + c]);
         const T rho = w[DIM];
         T vsq = T(0);
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
             T u = fdiv(w[k], rho);
-            s.qn[(size_t)k * g.NT + c] = u;
+            __builtin_nontemporal_store(u, &s.qn[(size_t)k * g.NT + c]);
             vsq += u * u;
         }
         T Tn = fdiv(fdiv(w[DIM + 1], rho) - (vsq * T(0.5)), T(ph.cv));
-        s.qn[(size_t)DIM * g.NT + c] = rho * T(ph.rm) * Tn;
-        s.qn[(size_t)(DIM + 1) * g.NT + c] = Tn;
-        s.qn[(size_t)(DIM + 2) * g.NT + c] = rho;
+        __builtin_nontemporal_store(rho * T(ph.rm) * Tn, &s.qn[(size_t)DIM * g.NT + c]);
+        __builtin_nontemporal_store(Tn, &s.qn[(size_t)(DIM + 1) * g.NT + c]);
+        __builtin_nontemporal_store(rho, &s.qn[(size_t)(DIM + 2) * g.NT + c]);
     }
     // ---- ghosts of the tile's boundary faces (psstFiniteVolumeGrid.cpp:351-366) ----
     __syncthreads();
