@@ -45,7 +45,8 @@ def parse():
     ap.add_argument("--numbering", default="rcm", choices=("rcm", "random", "column"))
     ap.add_argument("--time-mode", type=int, default=2)
     ap.add_argument("--fused", type=int, default=-1, help="-1: library default")
-    ap.add_argument("--cpu-iters", type=int, default=2, help="iterations of the CPU oracle sample (0: skip)")
+    ap.add_argument("--cpu-iters", type=int, default=3,
+                    help="iterations of the CPU oracle sample, about 14 s on one host core (0: skip)")
     ap.add_argument("--also-fp32", type=int, default=1, help="also time the FP32 mode")
     ap.add_argument("--timing-iters", type=int, default=10)
     ap.add_argument("--profile", action="store_true", help="short run for rocprof (no CPU baseline, no JSON extras)")

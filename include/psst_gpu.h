@@ -294,7 +294,9 @@ psst_status psst_case_write_dat(const psst_case *c, const double *prim, const ch
  * cells per zone, the trees of an adapted mesh, zone names
  * (psstAnsysOutputManager::private_write_cas_file, psstOutputManager.cpp:379-398). */
 psst_status psst_case_write_cas(const psst_case *c, const char *path);
-/* Replaces the settings of a case (e.g. to change the scheme or precision). */
+/* Replaces the settings of a case (e.g. to change the scheme or precision).
+ * A boundary table in `settings` replaces the case's entry for entry; one with
+ * a different n_bc is PSST_BAD_ARG and leaves the case unchanged. */
 psst_status psst_case_set_settings(psst_case *c, const psst_settings *settings);
 void psst_case_close(psst_case *c);
 const char *psst_case_last_error(void);

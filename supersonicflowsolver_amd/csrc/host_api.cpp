@@ -203,8 +203,10 @@ psst_status psst_case_settings(const psst_case *c, psst_settings *s) {
 
 psst_status psst_case_set_settings(psst_case *c, const psst_settings *s) {
     if (!c || !s) return PSST_BAD_ARG;
+    // a boundary table, when given, replaces the case's entry for entry
+    if (s->bc && s->bc != c->c.bc.data() && s->n_bc != (int32_t)c->c.bc.size()) return PSST_BAD_ARG;
     c->c.settings = *s;
-    if (s->bc && s->bc != c->c.bc.data() && s->n_bc == (int32_t)c->c.bc.size())
+    if (s->bc && s->bc != c->c.bc.data())
         std::memcpy(c->c.bc.data(), s->bc, sizeof(psst_bc) * c->c.bc.size());
     c->c.settings.bc = c->c.bc.data();
     c->c.settings.n_bc = (int32_t)c->c.bc.size();
@@ -244,29 +246,33 @@ psst_status psst_case_raw(const psst_case *c, double *xyz, int8_t *face_nv, int6
                           int64_t *face_zone, int64_t *cell_zone, int64_t *zone_id, int64_t *zone_range,
                           int64_t *zone_bc, char *zone_names, int32_t names_len) {
     if (!c) return PSST_BAD_ARG;
-    const psst::RawMesh &m = c->raw;
-    if (xyz) std::memcpy(xyz, m.xyz.data(), sizeof(double) * m.xyz.size());
-    if (face_nv) std::memcpy(face_nv, m.face_nv.data(), m.face_nv.size());
-    if (face_v) std::memcpy(face_v, m.face_v.data(), sizeof(int64_t) * m.face_v.size());
-    if (face_c) std::memcpy(face_c, m.face_c.data(), sizeof(int64_t) * m.face_c.size());
-    if (face_zone) std::memcpy(face_zone, m.face_zone.data(), sizeof(int64_t) * m.face_zone.size());
-    if (cell_zone) std::memcpy(cell_zone, m.cell_zone.data(), sizeof(int64_t) * m.cell_zone.size());
-    std::string names;
-    for (size_t i = 0; i < m.zones.size(); ++i) {
-        if (zone_id) zone_id[i] = m.zones[i].id;
-        if (zone_range) {
-            zone_range[2 * i] = m.zones[i].first;
-            zone_range[2 * i + 1] = m.zones[i].last;
+    try {
+        const psst::RawMesh &m = c->raw;
+        if (xyz) std::memcpy(xyz, m.xyz.data(), sizeof(double) * m.xyz.size());
+        if (face_nv) std::memcpy(face_nv, m.face_nv.data(), m.face_nv.size());
+        if (face_v) std::memcpy(face_v, m.face_v.data(), sizeof(int64_t) * m.face_v.size());
+        if (face_c) std::memcpy(face_c, m.face_c.data(), sizeof(int64_t) * m.face_c.size());
+        if (face_zone) std::memcpy(face_zone, m.face_zone.data(), sizeof(int64_t) * m.face_zone.size());
+        if (cell_zone) std::memcpy(cell_zone, m.cell_zone.data(), sizeof(int64_t) * m.cell_zone.size());
+        std::string names;
+        for (size_t i = 0; i < m.zones.size(); ++i) {
+            if (zone_id) zone_id[i] = m.zones[i].id;
+            if (zone_range) {
+                zone_range[2 * i] = m.zones[i].first;
+                zone_range[2 * i + 1] = m.zones[i].last;
+            }
+            if (zone_bc) zone_bc[i] = m.zones[i].bc;
+            names += m.zones[i].name;
+            names += '\n';
         }
-        if (zone_bc) zone_bc[i] = m.zones[i].bc;
-        names += m.zones[i].name;
-        names += '\n';
+        if (zone_names) {
+            if ((int32_t)names.size() + 1 > names_len) return PSST_BAD_ARG;
+            std::memcpy(zone_names, names.c_str(), names.size() + 1);
+        }
+        return PSST_OK;
+    } catch (const std::exception &) {  // no exception crosses the C ABI
+        return PSST_BAD_ARG;
     }
-    if (zone_names) {
-        if ((int32_t)names.size() + 1 > names_len) return PSST_BAD_ARG;
-        std::memcpy(zone_names, names.c_str(), names.size() + 1);
-    }
-    return PSST_OK;
 }
 
 psst_status psst_case_write_sav(const psst_case *c, const double *prim, int64_t iter_value, const char *path) {

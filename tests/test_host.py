@@ -96,6 +96,27 @@ def test_bad_inputs_raise(tmp_path):
         ss.Case.open(mesh, prefix)
 
 
+def test_settings_with_a_boundary_table_of_another_size_are_rejected(tmp_path):
+    mesh, prefix, _, _ = unpack("m_hll_o1_lc0_rk1", str(tmp_path))
+    case = ss.Case.open(mesh, prefix)
+    s = case.settings()
+    n = s.n_bc
+    table = (ss.BC * (n + 1))()
+    for i in range(n):
+        table[i] = s.bc[i]
+    table[n] = s.bc[0]
+    s.bc, s.n_bc = ctypes.cast(table, ctypes.POINTER(ss.BC)), n + 1
+    with pytest.raises(ss.PsstError):
+        case.set_settings(s)
+    # unchanged, and a table of the right size is taken entry for entry
+    t = case.settings()
+    assert t.n_bc == n
+    s.n_bc = n
+    s.cfl = 0.5
+    case.set_settings(s)
+    assert case.settings().cfl == 0.5
+
+
 def test_trailing_newline_in_boundary_files_is_accepted(tmp_path):
     """The reference crashes on a trailing newline in .bvp/.btq (quirk Q11);
     the host reader accepts it and reads the same boundary set."""
