@@ -1,0 +1,62 @@
+"""Physics checks on the CPU: the exact solutions the GPU physics tests
+compare with are pinned against published values, and the oracle itself
+solves the Sod shock tube to within the GPU tests' tolerances."""
+import math
+
+import numpy as np
+import pytest
+
+from _physics import SOD_ITERS, sod_case, sod_errors
+from _riemann import oblique_shock, riemann_exact, star_state
+from oracle.oracle import Oracle
+
+# Toro, "Riemann Solvers and Numerical Methods for Fluid Dynamics", 3rd ed.,
+# Table 4.1 (initial data, gamma = 1.4) and Table 4.3 (p*, u*, rho*_L, rho*_R)
+TORO = [
+    ((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 0.30313, 0.92745, 0.42632, 0.26557),
+    ((1.0, -2.0, 0.4), (1.0, 2.0, 0.4), 0.00189, 0.00000, 0.02185, 0.02185),
+    ((1.0, 0.0, 1000.0), (1.0, 0.0, 0.01), 460.894, 19.5975, 0.57506, 5.99924),
+    ((1.0, 0.0, 0.01), (1.0, 0.0, 100.0), 46.0950, -6.19633, 5.99242, 0.57511),
+    ((5.99924, 19.5975, 460.894), (5.99242, -6.19633, 46.0950), 1691.64, 8.68975, 14.2823, 31.0426),
+]
+
+
+@pytest.mark.parametrize("left,right,ps,us,rsl,rsr", TORO)
+def test_exact_riemann_solver_reproduces_toro_table(left, right, ps, us, rsl, rsr):
+    p, u = star_state(left, right, 1.4)
+    assert p == pytest.approx(ps, rel=2e-5, abs=1e-5)
+    assert u == pytest.approx(us, rel=2e-5, abs=1e-5)
+    # densities just either side of the contact
+    d = riemann_exact(left, right, 1.4, np.array([u - 1e-9, u + 1e-9]))
+    assert d[0, 0] == pytest.approx(rsl, rel=2e-5, abs=1e-5)
+    assert d[0, 1] == pytest.approx(rsr, rel=2e-5, abs=1e-5)
+    # far field is the initial data
+    far = riemann_exact(left, right, 1.4, np.array([-1e6, 1e6]))
+    assert tuple(far[:, 0]) == left and tuple(far[:, 1]) == right
+
+
+@pytest.mark.parametrize("mach,theta,beta", [(2.0, 10.0, 39.31), (2.5, 15.0, 36.94), (3.0, 20.0, 37.76)])
+def test_oblique_shock_relation_known_values(mach, theta, beta):
+    """Weak-shock angles from the theta-beta-M charts (NACA Report 1135);
+    the jumps are the normal-shock relations at M sin(beta)."""
+    b, p21, r21 = oblique_shock(mach, theta, 1.4)
+    assert b == pytest.approx(beta, abs=0.01)
+    mn = mach * math.sin(math.radians(b))
+    assert p21 == pytest.approx((2.8 * mn * mn - 0.4) / 2.4, rel=1e-12)
+    assert r21 == pytest.approx(2.4 * mn * mn / (0.4 * mn * mn + 2.0), rel=1e-12)
+    # a vanishing wedge gives a Mach wave and no jump
+    b0, p0, _ = oblique_shock(mach, 1e-6, 1.4)
+    assert b0 == pytest.approx(math.degrees(math.asin(1.0 / mach)), abs=1e-3)
+    assert p0 == pytest.approx(1.0, abs=1e-6)
+
+
+@pytest.mark.parametrize("flux", [3, 2, 1])
+def test_oracle_solves_the_sod_shock_tube(tmp_path, flux):
+    mesh, prefix, case, prim, x = sod_case(str(tmp_path), flux)
+    o = Oracle(mesh, prefix)
+    o.set_state(prim)
+    o.run(SOD_ITERS)
+    e = sod_errors(case, x, o.state())
+    assert e["rho"] < 3e-3 and e["p"] < 2e-3 and e["u"] < 3e-3, e
+    assert abs(e["p_star"]) < 1e-3, e
+    assert e["v_max"] < 1e-6, e
