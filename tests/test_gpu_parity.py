@@ -294,6 +294,54 @@ def test_restart_from_the_reference_sav(tmp_path, name):
     assert _sha(prefix + ".dat") == meta["reference_output_sha256"]["dat"]
 
 
+def _read_sav(path):
+    """(iteration, rows of p T u_i) of a .sav file."""
+    with open(path) as f:
+        it = int(f.readline())
+        f.readline()  # zone id
+        n = int(f.readline())
+        rows = np.array([[float(v) for v in f.readline().split()] for _ in range(n)])
+    return it, rows
+
+
+def test_restart_after_a_save_continues_the_run(tmp_path):
+    """A run stopped right after a save and restarted from its .sav (.inp
+    init mode 1) ends where the uninterrupted run ends.  The .sav keeps 11
+    decimals, so the restarted fields agree to rounding, not bitwise; the
+    residual file holds the first run's rows, then the restart's."""
+    mesh = str(tmp_path / "w.msh")
+    ss.write_wedge_msh(ss.wedge_params(nx=60, ny=20, jitter=0.1), mesh)
+    opts = dict(cfl=0.8, time_mode=2, rk_levels=4, flux=3, order=2, grad=1, limiter=2)
+    full = str(tmp_path / "full")
+    ss.write_case_files(full, iterations=60, save_step=30, **opts)
+    ss.Case.open(mesh, full).solve()
+    part = str(tmp_path / "part")
+    ss.write_case_files(part, iterations=30, save_step=30, **opts)
+    ss.Case.open(mesh, part).solve()
+    it_mid, _ = _read_sav(part + ".sav")
+    ss.write_case_files(part, iterations=30, save_step=30, init_mode=1, **opts)
+    case = ss.Case.open(mesh, part)
+    assert case.first_iter == it_mid
+    case.solve()
+    it_a, a = _read_sav(full + ".sav")
+    it_b, b = _read_sav(part + ".sav")
+    assert it_a == it_b and a.shape == b.shape
+    rel = np.abs(a - b).max(axis=0) / np.abs(a).max(axis=0)
+    print("restart: iteration", it_b, "max relative difference per column (p, T, u, v)", rel, flush=True)
+    assert rel.max() < 1e-9
+    # residual rows: iteration, wall time (left out), dim+2 norms
+    with open(full + "_residual.dat") as f:
+        ra = [l.split() for l in f.read().splitlines()]
+    with open(part + "_residual.dat") as f:
+        rb = [l.split() for l in f.read().splitlines()]
+    assert len(ra) == len(rb) == 60
+    assert [r[:1] + r[2:] for r in ra[:30]] == [r[:1] + r[2:] for r in rb[:30]]
+    assert [r[0] for r in rb] == [str(i) for i in range(1, 61)]
+    va = np.array([[float(v) for v in r[2:]] for r in ra[30:]])
+    vb = np.array([[float(v) for v in r[2:]] for r in rb[30:]])
+    assert np.allclose(va, vb, rtol=1e-2, atol=0)  # three printed digits
+
+
 def test_case_solve_reproduces_the_reference_run(tmp_path):
     """The reference's main() on its own case (wedge80_240.msh, wedge.inp: 3000
     iterations, time step mode 1 in the reference's order, a save every 1000)
