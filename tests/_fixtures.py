@@ -18,6 +18,8 @@ BC_CASES = sorted(d for d in (os.listdir(GOLDEN) if os.path.isdir(GOLDEN) else [
 RESTART_CASES = sorted(d for d in (os.listdir(GOLDEN) if os.path.isdir(GOLDEN) else []) if d.startswith("restart_"))
 # adapted (hanging-node) meshes of the reference: parent zones, face/cell trees
 ADAPTED_CASES = sorted(d for d in (os.listdir(GOLDEN) if os.path.isdir(GOLDEN) else []) if d.startswith("a_"))
+# triangles: the wedge channel with every quadrilateral split in two
+TRIANGLE_CASES = sorted(d for d in (os.listdir(GOLDEN) if os.path.isdir(GOLDEN) else []) if d.startswith("t_"))
 
 
 def unpack(name, workdir):

@@ -19,6 +19,9 @@ Fixture kinds:
   restart_*          a restart from the reference's own .sav (init mode 1).
   b_*                the subsonic outlet (25) and isothermal walls (11) on the
                      10x30 wedge, three schemes.
+  t_*                triangles: the 30x10 wedge channel with every quadrilateral
+                     split in two (written by write_triangle_wedge_msh, stored
+                     once under _meshes/), three schemes, 12 iterations each.
   a_*                adapted (hanging-node) meshes: the reference's wedge3.msh,
                      wedge5.msh and wedge_adaptation.msh (Fluent 16 files with
                      parent zones and face/cell trees; stored once under
@@ -154,15 +157,21 @@ def committed_output_digests():
         json.dump(meta, f, indent=1, sort_keys=True)
 
 
-def generated_case(name, params, case_kw, iterations, mesh_src=None, shared_mesh=None):
+def generated_case(name, params, case_kw, iterations, mesh_src=None, shared_mesh=None, mesh_writer=None,
+                   mesh_source=None):
     """shared_mesh: store the mesh once under _meshes/<shared_mesh>.msh.xz
-    (several fixtures on the same mesh) instead of in the fixture dir."""
+    (several fixtures on the same mesh) instead of in the fixture dir.
+    mesh_writer: a function writing the mesh file (instead of the package's
+    wedge generator), described by mesh_source."""
     import supersonicflowsolver_amd as ss
     d = os.path.join(HERE, name)
     os.makedirs(d, exist_ok=True)
     dim = case_kw.get("dim", 2)
     with tempfile.TemporaryDirectory() as w:
-        if mesh_src is None:
+        if mesh_writer is not None:
+            mesh_path = os.path.join(w, "mesh.msh")
+            mesh_writer(mesh_path)
+        elif mesh_src is None:
             mesh_path = os.path.join(w, "mesh.msh")
             ss.write_wedge_msh(ss.wedge_params(**params), mesh_path)
         else:
@@ -184,7 +193,7 @@ def generated_case(name, params, case_kw, iterations, mesh_src=None, shared_mesh
         vals, res = run_reference(w, "mesh.msh", "case")
         digests = output_digests(w, "case")
     meta = {"iterations": iterations, "wedge_params": params, "case": kw,
-            "mesh_source": ("reference " + os.path.relpath(mesh_src, REF)) if mesh_src
+            "mesh_source": mesh_source if mesh_source else ("reference " + os.path.relpath(mesh_src, REF)) if mesh_src
             else "supersonicflowsolver_amd wedge generator",
             "reference_output_sha256": digests}
     if shared_mesh:
@@ -313,6 +322,84 @@ def restart_case():
                                     "reference_output_sha256": digests})
 
 
+def write_triangle_wedge_msh(path, nx=30, ny=10, ramp_deg=15.0, jitter=0.1, seed=7):
+    """The wedge channel (x in [0, 3], 15 degree ramp on x in [0.5, 1.1]) on
+    an nx x ny grid of quadrilaterals, each split along its rising diagonal
+    into two triangles; interior nodes jittered by up to `jitter` of the local
+    spacing.  Fluent ASCII, faces "v0 v1 c0 c1" (hexadecimal) with the normal
+    (y0 - y1, x1 - x0)/A pointing toward c0, ghosts as c0 = 0.  Zones: 3 the
+    bottom ("wall"), 4 "out", 5 "up", 6 "in", 10 the interior faces."""
+    import math
+    rng = np.random.default_rng(seed)
+    t = math.tan(math.radians(ramp_deg))
+
+    def y_bottom(x):
+        return 0.0 if x < 0.5 else (x - 0.5) * t if x <= 1.1 else 0.6 * t
+
+    xy = np.zeros((ny + 1, nx + 1, 2))
+    for j in range(ny + 1):
+        for i in range(nx + 1):
+            x = 3.0 * i / nx
+            yb = y_bottom(x)
+            y = yb + (1.0 - yb) * j / ny
+            if 0 < i < nx and 0 < j < ny:
+                x += jitter * (3.0 / nx) * rng.uniform(-1.0, 1.0)
+                y += jitter * ((1.0 - yb) / ny) * rng.uniform(-1.0, 1.0)
+            xy[j, i] = (x, y)
+
+    def v(i, j):
+        return 1 + j * (nx + 1) + i
+
+    def t0(i, j):  # lower-right triangle of quad (i, j)
+        return 1 + 2 * (j * nx + i)
+
+    def t1(i, j):  # upper-left triangle
+        return 2 + 2 * (j * nx + i)
+
+    zones = [
+        (3, 3, [(v(i + 1, 0), v(i, 0), 0, t0(i, 0)) for i in range(nx)]),
+        (4, 3, [(v(nx, j + 1), v(nx, j), 0, t0(nx - 1, j)) for j in range(ny)]),
+        (5, 3, [(v(i, ny), v(i + 1, ny), 0, t1(i, ny - 1)) for i in range(nx)]),
+        (6, 3, [(v(0, j), v(0, j + 1), 0, t1(0, j)) for j in range(ny)]),
+        (10, 2, [(v(i + 1, j + 1), v(i, j), t0(i, j), t1(i, j)) for j in range(ny) for i in range(nx)]
+         + [(v(i, j), v(i + 1, j), t0(i, j), t1(i, j - 1)) for j in range(1, ny) for i in range(nx)]
+         + [(v(i, j), v(i, j + 1), t0(i - 1, j), t1(i, j)) for j in range(ny) for i in range(1, nx)]),
+    ]
+    nc, nv = 2 * nx * ny, (nx + 1) * (ny + 1)
+    nf = sum(len(z[2]) for z in zones)
+    out = ['(0 "Grid")', "", '(0 "Dimension:")', "(2 2)", "", '(0 "Cells:")', "(12 (0 1 %x 0))" % nc,
+           "(12 (2 1 %x 1 1))" % nc, "", "", '(0 "Faces:")', "(13 (0 1 %x 0))" % nf]
+    first = 1
+    for zid, bc, faces in zones:
+        out.append("(13 (%x %x %x %x 2)(" % (zid, first, first + len(faces) - 1, bc))
+        out += [" %x %x %x %x" % f for f in faces]
+        out += [")", ")"]
+        first += len(faces)
+    out += ["", '(0 "Nodes:")', "(10 (0 1 %x 1 2))" % nv, "(10 (1 1 %x 1 2)(" % nv]
+    out += [" %.20e  %.20e" % (xy[j, i, 0], xy[j, i, 1]) for j in range(ny + 1) for i in range(nx + 1)]
+    out += [")", ")", "", '(0 "Zones:")', "(39 (2 fluid fluid)())", "(39 (3 wall wall)())", "(39 (4 wall out)())",
+            "(39 (5 wall up)())", "(39 (6 wall in)())", "(39 (10 interior default-interior)())", ""]
+    with open(path, "w") as f:
+        f.write("\n".join(out))
+
+
+TRIANGLES = [
+    # name, flux, order, grad, limiter, time_mode, rk
+    ("t_hllc_o2_gg_mm_lc2_rk4", 3, 2, 1, 2, 2, 4),
+    ("t_ausm_o2_lsq_vl_lc1_rk3", 1, 2, 2, 3, 1, 3),
+    ("t_hll_o2_gg_va_lc0_rk2", 2, 2, 1, 1, 0, 2),
+]
+
+
+def triangles():
+    """Triangles: the wedge channel with every quadrilateral split in two."""
+    for name, flux, order, grad, lim, lc, rk in TRIANGLES:
+        kw = dict(cfl=0.5, rk_levels=rk, flux=flux, order=order, grad=grad, limiter=lim, time_mode=lc, mach=2.5,
+                  save_step=100000, wall_names=("wall", "up"))
+        generated_case(name, None, kw, 12, shared_mesh="tri_wedge30x10", mesh_writer=write_triangle_wedge_msh,
+                       mesh_source="triangulated 30x10 wedge, write_triangle_wedge_msh in tests/golden/make_golden.py")
+
+
 def adapted():
     for name, mesh, flux, order, grad, lim, lc, rk, cfl in ADAPTED:
         kw = dict(ADAPTED_CASE, cfl=cfl, rk_levels=rk, flux=flux, order=order, grad=grad, limiter=lim,
@@ -333,6 +420,8 @@ def main():
         return restart_case()
     if sys.argv[1:] == ["customrk"]:
         return custom_rk_case()
+    if sys.argv[1:] == ["triangles"]:
+        return triangles()
     if sys.argv[1:] == ["digests"]:
         committed_output_digests()
         return fixture_output_digests()

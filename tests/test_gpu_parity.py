@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import supersonicflowsolver_amd as ss
-from _fixtures import ADAPTED_CASES, BC_CASES, GENERATED_CASES, RESTART_CASES, MATRIX_CASES, THREE_D_CASES, residual_lines, unpack
+from _fixtures import ADAPTED_CASES, BC_CASES, TRIANGLE_CASES, GENERATED_CASES, RESTART_CASES, MATRIX_CASES, THREE_D_CASES, residual_lines, unpack
 from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -58,7 +58,7 @@ def test_kernel_parity_gradients_and_residual(tmp_path, name):
 
 
 @pytest.mark.parametrize("fused", [1, 0])
-@pytest.mark.parametrize("name", GENERATED_CASES + MATRIX_CASES + THREE_D_CASES + ADAPTED_CASES + BC_CASES)
+@pytest.mark.parametrize("name", GENERATED_CASES + MATRIX_CASES + THREE_D_CASES + ADAPTED_CASES + BC_CASES + TRIANGLE_CASES)
 def test_full_run_matches_reference_build(tmp_path, name, fused):
     """Whole run (fixture iteration count): final state bit-identical to the
     reference build's, residual history bit-identical to the oracle's and equal

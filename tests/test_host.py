@@ -86,6 +86,106 @@ def test_generator_mesh_is_closed_and_oriented(tmp_path, numbering):
     assert ((d * fn[:2]).sum(axis=0) > 0).all()
 
 
+# Two triangles filling the unit square: cell 1 = (0,0) (1,0) (1,1), cell 2 =
+# (0,0) (1,1) (0,1).  Faces are "v0 v1 c0 c1" with the normal
+# (y0 - y1, x1 - x0)/A pointing toward c0 (the ghost side on the boundary).
+TRIANGLES_MSH = """(0 "two triangles")
+(2 2)
+(12 (0 1 2 0))
+(12 (2 1 2 1 1))
+(13 (0 1 5 0))
+(13 (3 1 1 3 2)(
+ 2 1 0 1
+)
+)
+(13 (4 2 2 3 2)(
+ 3 2 0 1
+)
+)
+(13 (5 3 3 3 2)(
+ 4 3 0 2
+)
+)
+(13 (6 4 4 3 2)(
+ 1 4 0 2
+)
+)
+(13 (a 5 5 2 2)(
+ 3 1 1 2
+)
+)
+(10 (0 1 4 1 2))
+(10 (1 1 4 1 2)(
+ 0.0 0.0
+ 1.0 0.0
+ 1.0 1.0
+ 0.0 1.0
+)
+)
+(39 (2 fluid fluid)())
+(39 (3 wall wall)())
+(39 (4 wall out)())
+(39 (5 wall up)())
+(39 (6 wall in)())
+(39 (10 interior default-interior)())
+"""
+
+
+def test_triangle_geometry_known_answers(tmp_path):
+    """Host geometry on triangles against closed forms: areas 1/2, centroids
+    at the vertex means, the diagonal's length sqrt(2) and unit normal toward
+    neighbour 0, outward unit normals on the boundary, ghosts of zero volume
+    at their face centres, a Green-Gauss weight of 1/2 on the symmetric
+    diagonal, and closed cells (sum of n A = 0)."""
+    mesh = str(tmp_path / "tri.msh")
+    with open(mesh, "w") as f:
+        f.write(TRIANGLES_MSH)
+    prefix = str(tmp_path / "c")
+    ss.write_case_files(prefix, wall_names=("wall", "up"))
+    case = ss.Case.open(mesh, prefix)
+    N, G, F = case.n_cells, case.n_ghosts, case.n_faces
+    assert (case.dim, N, G, F) == (2, 2, 4, 5)
+    m = case.mesh()
+    NT = N + G
+    vol = np.ctypeslib.as_array(m.cell_volume, shape=(NT,))
+    cx = np.ctypeslib.as_array(m.cell_centre, shape=(3 * NT,)).reshape(3, NT)[:2]
+    fn = np.ctypeslib.as_array(m.face_normal, shape=(3 * F,)).reshape(3, F)[:2]
+    fa = np.ctypeslib.as_array(m.face_area, shape=(F,))
+    fc = np.ctypeslib.as_array(m.face_cells, shape=(2 * F,)).reshape(2, F)
+    beta = np.ctypeslib.as_array(m.face_beta, shape=(F,))
+    assert np.allclose(vol[:N], 0.5, rtol=0, atol=1e-15) and (vol[N:] == 0).all()
+    assert m.min_volume == pytest.approx(0.5, abs=1e-15)
+    cents = cx[:, :N].T[np.argsort(cx[0, :N])]
+    assert np.allclose(cents, [(1 / 3, 2 / 3), (2 / 3, 1 / 3)], rtol=0, atol=1e-15)
+    inner = [f for f in range(F) if fc[0, f] < N and fc[1, f] < N]
+    assert len(inner) == 1
+    d = inner[0]
+    assert fa[d] == pytest.approx(np.sqrt(2.0), abs=1e-15)
+    c0, c1 = fc[0, d], fc[1, d]
+    assert np.dot(fn[:, d], cx[:, c0] - cx[:, c1]) > 0
+    assert np.allclose(np.abs(fn[:, d]), np.sqrt(0.5), rtol=0, atol=1e-15)
+    assert beta[d] == pytest.approx(0.5, abs=1e-14)
+    # boundary faces: unit length, outward normal, ghost at the face centre
+    expected = {(0.5, 0.0): (0.0, -1.0), (1.0, 0.5): (1.0, 0.0), (0.5, 1.0): (0.0, 1.0), (0.0, 0.5): (-1.0, 0.0)}
+    for f in range(F):
+        if f == d:
+            continue
+        ghost = fc[0, f] if fc[0, f] >= N else fc[1, f]
+        centre = tuple(np.round(cx[:, ghost], 14) + 0.0)
+        assert centre in expected, centre
+        assert fa[f] == pytest.approx(1.0, abs=1e-15)
+        assert np.allclose(fn[:, f], expected[centre], rtol=0, atol=1e-15)
+    acc = np.zeros((2, NT))
+    for k in range(2):
+        np.add.at(acc[k], fc[0], -fn[k] * fa)
+        np.add.at(acc[k], fc[1], fn[k] * fa)
+    assert np.abs(acc[:, :N]).max() < 1e-15
+    # the oracle reads the same mesh to the same set-up
+    o = Oracle(mesh, prefix)
+    assert (o.n_cells, o.n_ghosts, o.n_faces) == (N, G, F) and o.min_vol == m.min_volume
+    assert np.array_equal(case.initial_state(), o.state())
+
+
 def test_bad_inputs_raise(tmp_path):
     with pytest.raises(ss.PsstError):
         ss.Case.open(str(tmp_path / "missing.msh"), str(tmp_path / "c"))

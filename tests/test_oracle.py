@@ -19,7 +19,7 @@ import os
 import numpy as np
 import pytest
 
-from _fixtures import ADAPTED_CASES, BC_CASES, GENERATED_CASES, RESTART_CASES, MATRIX_CASES, THREE_D_CASES, residual_lines, unpack
+from _fixtures import ADAPTED_CASES, BC_CASES, TRIANGLE_CASES, GENERATED_CASES, RESTART_CASES, MATRIX_CASES, THREE_D_CASES, residual_lines, unpack
 from oracle.oracle import Oracle
 
 
@@ -53,7 +53,7 @@ def test_reference_build_dump_80x240_100it(tmp_path):
     assert residual_lines(res) == exp["residual"]
 
 
-@pytest.mark.parametrize("name", GENERATED_CASES + MATRIX_CASES + THREE_D_CASES + ADAPTED_CASES + BC_CASES)
+@pytest.mark.parametrize("name", GENERATED_CASES + MATRIX_CASES + THREE_D_CASES + ADAPTED_CASES + BC_CASES + TRIANGLE_CASES)
 def test_reference_build_dumps(tmp_path, name):
     mesh, prefix, exp, meta = unpack(name, str(tmp_path))
     o = Oracle(mesh, prefix)

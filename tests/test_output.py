@@ -16,7 +16,7 @@ import hashlib
 import pytest
 
 import supersonicflowsolver_amd as ss
-from _fixtures import ADAPTED_CASES, BC_CASES, GENERATED_CASES, MATRIX_CASES, RESTART_CASES, THREE_D_CASES, unpack
+from _fixtures import ADAPTED_CASES, BC_CASES, TRIANGLE_CASES, GENERATED_CASES, MATRIX_CASES, RESTART_CASES, THREE_D_CASES, unpack
 from oracle.oracle import Oracle
 
 
@@ -25,7 +25,7 @@ def _sha(path):
         return hashlib.sha256(f.read()).hexdigest()
 
 
-@pytest.mark.parametrize("name", GENERATED_CASES + MATRIX_CASES + THREE_D_CASES + ADAPTED_CASES + BC_CASES +
+@pytest.mark.parametrize("name", GENERATED_CASES + MATRIX_CASES + THREE_D_CASES + ADAPTED_CASES + BC_CASES + TRIANGLE_CASES +
                          RESTART_CASES)
 def test_cas_and_dat_equal_the_reference_builds_files(tmp_path, name):
     mesh, prefix, exp, meta = unpack(name, str(tmp_path))
