@@ -230,7 +230,7 @@ def alg_bytes(kind, N, G, F, nnz, dim, b, last_stage=False):
     raise ValueError(kind)
 
 
-def survey_bytes_per_cell_iteration(dim, b, N, F, stages=4):
+def model_bytes_per_cell_iteration(dim, b, N, F, stages=4):
     """Unique-touch model per cell-iteration (second order, RK): per stage
     q read + grad read + w0 read + q write + V, x; faces n, A, c0, c1; plus
     the gradient pass and the w0 snapshot."""
@@ -360,7 +360,7 @@ def main():
                 "alg_bytes_per_launch": int(nbytes), "mean_ms": round(mean_ms, 4)}
     if note:
         roofline["note"] = note
-    whole = survey_bytes_per_cell_iteration(dim, b, N, F)
+    whole = model_bytes_per_cell_iteration(dim, b, N, F)
     out = {
         "metric": METRIC, "value": round(cell_its, 1), "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(el / a.steps * 1e3, 4), "higher_is_better": True,
@@ -384,7 +384,7 @@ def main():
     if a.also_fp32 and a.precision == 64:
         el32, stats32, state32 = timed(32)
         ci32 = N * a.steps / el32
-        whole32 = survey_bytes_per_cell_iteration(dim, 4, N, F)
+        whole32 = model_bytes_per_cell_iteration(dim, 4, N, F)
         # relative L2 per field; the velocity components against the norm of
         # the speed (a component that is zero in FP64, such as w on the
         # extruded wedge, has no scale of its own)

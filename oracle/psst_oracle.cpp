@@ -18,7 +18,7 @@
  *   - whole-run fields: bit-identical to a g++ build of the reference
  *     (oracle/Makefile target `ref`) on the committed fixtures in
  *     tests/golden/ (80x240 HLL/RK1/lc=1 reference-held run; generated
- *     40x120 C1 runs; a small matrix of schemes; the reference's 3D hex
+ *     40x120 wedge runs; a small matrix of schemes; the reference's 3D hex
  *     meshes; its adapted meshes with parent zones and face/cell trees);
  *   - `.sav` and residual file of the reference-held run byte-compared;
  *   - limiters, Roe average, fluxes, geometry, gradients also pinned by

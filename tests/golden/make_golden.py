@@ -431,7 +431,7 @@ def main():
     for name, flux, order, grad, lim, lc, rk in MATRIX:
         generated_case(name, SMALL, dict(cfl=0.6, rk_levels=rk, flux=flux, order=order, grad=grad, limiter=lim,
                                          time_mode=lc, mach=2.5, save_step=100000), 12)
-    # the reference's own 40x120 mesh (18.4 degree ramp) with the C1 scheme
+    # the reference's own 40x120 mesh (18.4 degree ramp) with the 40x120 wedge scheme
     generated_case("refmesh40x120_lc2", None, dict(C1_CASE, time_mode=2), 200,
                    mesh_src=os.path.join(REF, "wedge40_120.msh"))
     three_d()

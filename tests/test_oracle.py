@@ -5,7 +5,7 @@ Pins:
   * the reference's committed golden run (80x240, HLL, first order, RK1,
     lc=1, 3000 iterations): .sav byte-identical, every residual row equal;
   * g++ builds of the reference (tests/golden/*/expected.npz): final states
-    bit-identical, residual files equal, for the C1 setups (HLLC, DM/MinMod,
+    bit-identical, residual files equal, for the 40x120 wedge set-ups (HLLC, DM/MinMod,
     Green-Gauss, RK4, lc 1 and 2, 2000 iterations) and a matrix of schemes
     (HLL/HLLC/AUSM x order 1/2 x GG/LSQ x three limiters x lc 0/1/2 x RK1..5),
     3D hexahedral runs on the reference's wedge3D.msh and quad3D.msh, and
