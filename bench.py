@@ -17,6 +17,7 @@ the CPU oracle's serial throughput on a bounded sample of the same workload.
 import argparse
 import json
 import os
+import shutil
 import sys
 import tempfile
 import time
@@ -47,6 +48,8 @@ def parse():
     ap.add_argument("--fused", type=int, default=-1, help="-1: library default")
     ap.add_argument("--cpu-iters", type=int, default=3,
                     help="iterations of the CPU oracle sample, about 14 s on one host core (0: skip)")
+    ap.add_argument("--ref-iters", type=int, default=5,
+                    help="iterations of the reference build's sample (a 4x coarser wedge, about 12 s; 0: skip)")
     ap.add_argument("--also-fp32", type=int, default=1, help="also time the FP32 mode")
     ap.add_argument("--timing-iters", type=int, default=10)
     ap.add_argument("--profile", action="store_true", help="short run for rocprof (no CPU baseline, no JSON extras)")
@@ -230,6 +233,54 @@ def alg_bytes(kind, N, G, F, nnz, dim, b, last_stage=False):
     raise ValueError(kind)
 
 
+def reference_cpu_sample(ss, a, numbering):
+    """The reference's own serial solver (oracle/_ref/psst_ref, g++ -O2 from
+    its sources) on a bounded sample of the 2D workload: the same wedge,
+    scheme and numbering at a quarter of the cells per direction, a.ref_iters
+    iterations.  Per-iteration wall time comes from the reference's residual
+    file, whose second column is the solve's elapsed steady_clock time in ms
+    resolution (psstFiniteVolumeGrid.cpp:278, :371-383): (t_last - t_first) /
+    (iterations - 1), so mesh reading, set-up and the final save are not
+    timed.  Returns a cpu_baseline object, or {"error": ...}."""
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "psst_ref")
+    if not os.path.exists(exe):
+        return {"error": "oracle/_ref/psst_ref is not built"}
+    nx, ny, n_it = a.nx // 4, a.ny // 4, a.ref_iters
+    d = tempfile.mkdtemp(prefix="psst_ref_")
+    ss.write_wedge_msh(ss.wedge_params(nx=nx, ny=ny, ramp_deg=15.0, jitter=0.1, jitter_seed=1, numbering=numbering,
+                                       perm_seed=2), os.path.join(d, "mesh.msh"))
+    ss.write_case_files(os.path.join(d, "case"), cfl=0.8, time_mode=a.time_mode, iterations=n_it,
+                        save_step=10 ** 9, rk_levels=4, flux=FLUX_IDS[a.flux], order=2, grad=1, limiter=2, mach=2.5,
+                        p=101325.0, T=300.0)
+    with open(os.path.join(d, "problem_manager"), "w") as f:
+        f.write("mesh.msh\ncase\n")
+    print("bench: reference build, %d-cell sample, %d iterations" % (nx * ny, n_it), file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    try:
+        subprocess.run([exe], cwd=d, stdin=subprocess.DEVNULL, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL, timeout=300, check=True)
+        with open(os.path.join(d, "case_residual.dat")) as f:
+            ts = [float(l.split()[1]) for l in f if l.strip()][:n_it]
+    except Exception as e:  # reported, not fatal: the port's figure stands in
+        return {"error": "reference build run failed: %s" % e}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    wall = time.perf_counter() - t
+    per_it = (ts[-1] - ts[0]) / (len(ts) - 1)
+    print("bench: reference build done (%.1f s wall, %.3f s per iteration)" % (wall, per_it), file=sys.stderr,
+          flush=True)
+    try:
+        cpu_model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+    except Exception:
+        cpu_model = "unknown"
+    return {"value": round(nx * ny / per_it, 1), "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+            "sample": "the reference's own solver (its sources, g++ -O2) on the same wedge, scheme and numbering at "
+                      "%dx%d (%d cells, 1/16 of the workload), iterations 2-%d of %d timed by its residual file's "
+                      "clock (mesh reading, set-up and output not timed); one host core (%s)"
+                      % (nx, ny, nx * ny, len(ts), n_it, cpu_model)}
+
+
 def model_bytes_per_cell_iteration(dim, b, N, F, stages=4):
     """Unique-touch model per cell-iteration (second order, RK): per stage
     q read + grad read + w0 read + q write + V, x; faces n, A, c0, c1; plus
@@ -279,6 +330,12 @@ def main():
     nnz = int(case.mesh().visit_offsets[N])
     print("bench: %s mesh, %d cells, %d faces, host setup %.1f s" % ("3D" if three_d else "2D", N, F, t_setup),
           file=sys.stderr, flush=True)
+
+    # the reference itself (compiled from its own sources into oracle/_ref),
+    # run as a child process before this process touches the GPU
+    ref_cpu = None
+    if a.ref_iters > 1 and not three_d and not a.profile:
+        ref_cpu = reference_cpu_sample(ss, a, numbering)
 
     if a.profile and a.calib_copy_mb > 0:
         import ctypes
@@ -411,11 +468,16 @@ def main():
             cpu_model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
         except Exception:
             cpu_model = "unknown"
-        out["cpu_baseline"] = {"value": round(N * a.cpu_iters / el_cpu, 1), "unit": "cell-updates/s", "cores": 1,
-                               "kind": "port",
-                               "sample": "%d iterations of the same %d-cell FP64 workload, serial oracle on one "
-                                         "host core (%s); setup %.1f s not timed" % (a.cpu_iters, N, cpu_model,
-                                                                                      t_open)}
+        out["cpu_port"] = {"value": round(N * a.cpu_iters / el_cpu, 1), "unit": "cell-updates/s", "cores": 1,
+                           "kind": "port",
+                           "sample": "%d iterations of the same %d-cell FP64 workload, serial oracle on one "
+                                     "host core (%s); setup %.1f s not timed" % (a.cpu_iters, N, cpu_model, t_open)}
+    if ref_cpu and "value" in ref_cpu:
+        out["cpu_baseline"] = ref_cpu
+    elif "cpu_port" in out:
+        out["cpu_baseline"] = out["cpu_port"]
+        if ref_cpu:
+            out["cpu_reference_error"] = ref_cpu["error"]
     print(json.dumps(out))
 
 
