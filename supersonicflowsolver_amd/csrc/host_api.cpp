@@ -127,7 +127,7 @@ class psstGridFV {
     void save_data(int64_t iter_value) {
         std::vector<double> prim(case_.prim.size());
         check(psst_get_state(ctx_, prim.data()));
-        psst::write_dat(raw_, case_.grid, prim.data(), case_.prefix + ".dat");
+        psst::write_dat(case_.grid, prim.data(), case_.prefix + ".dat");
         psst::write_cas(raw_, case_.prefix + ".cas");
         psst::write_sav(case_, prim.data(), iter_value, case_.prefix + ".sav");
     }
@@ -289,7 +289,7 @@ psst_status psst_case_write_sav(const psst_case *c, const double *prim, int64_t 
 psst_status psst_case_write_dat(const psst_case *c, const double *prim, const char *path) {
     if (!c || !prim || !path) return PSST_BAD_ARG;
     try {
-        psst::write_dat(c->raw, c->c.grid, prim, path);
+        psst::write_dat(c->c.grid, prim, path);
         return PSST_OK;
     } catch (const std::exception &e) {
         g_case_error = e.what();

@@ -67,7 +67,7 @@ std::ofstream open_append(const std::string &path) {
 
 }  // namespace
 
-void write_dat(const RawMesh &m, const Grid &g, const double *prim, const std::string &path) {
+void write_dat(const Grid &g, const double *prim, const std::string &path) {
     {
         std::ofstream trunc(path);
         if (!trunc.is_open()) throw std::runtime_error("Error: Could not open output *.dat file!");

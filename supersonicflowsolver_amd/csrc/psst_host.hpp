@@ -147,7 +147,7 @@ void write_sav(const Case &c, const double *prim, int64_t iter_value, const std:
 // one line of `<case>_residual.dat` (psstFiniteVolumeGrid.cpp:369-396)
 std::string residual_line(int64_t iter, double seconds, const double *res, int n);
 // `.dat` field file of a primitive state [(dim+3)][N+G] (psstOutputManager.cpp:71-128)
-void write_dat(const RawMesh &m, const Grid &g, const double *prim, const std::string &path);
+void write_dat(const Grid &g, const double *prim, const std::string &path);
 // `.cas` mesh file (psstOutputManager.cpp:130-398)
 void write_cas(const RawMesh &m, const std::string &path);
 
