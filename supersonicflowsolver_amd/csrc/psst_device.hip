@@ -512,7 +512,6 @@ __global__ void __launch_bounds__(kBlock) k_flux(Geom g, State<T> s, Phys ph) {
 #pragma unroll
             for (int k = 0; k < DIM; ++k) RLR[k] = T(g.rlr[(size_t)k * g.F + f]);
             const T *gr = s.grad;
-            const size_t N = (size_t)g.N;
             reconstruct_dm<DIM, T, LIM>(
                 r[0], r[1], RLR, T(ph.rm), T(ph.venkat_eps2),
                 [&](int i, int k) { return gr[(size_t)a0 * GradLayout<DIM>::STRIDE + i * DIM + k]; },
