@@ -63,6 +63,10 @@ def parse():
                     help="wedge: the 4.9M-cell iteration throughput; sweep: batched solves of many wedges to a "
                          "residual tolerance")
     ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--sweep-tune", default="largest", choices=("median", "largest"),
+                    help="sweep: the instance the autotuner times (the largest is bandwidth-bound like the batch; "
+                         "the median one, about 0.1 ms per iteration, is launch-bound)")
+    ap.add_argument("--tune-iters", type=int, default=30, help="sweep: autotuner trial iterations per tile shape")
     ap.add_argument("--sweep-seed", type=int, default=3)
     ap.add_argument("--cells-min", type=int, default=20000)
     ap.add_argument("--cells-max", type=int, default=400000)
@@ -169,11 +173,15 @@ def run_sweep(a):
     N = np.array([c.n_cells for c in cases], dtype=np.int64)
     log("%d instances, %d cells (min %d, max %d), host setup %.1f s" %
         (len(cases), N.sum(), N.min(), N.max(), time.perf_counter() - t0))
-    rep = int(np.argsort(N)[len(N) // 2])
-    t0 = time.perf_counter()
-    best, table = ss.autotune(cases[rep], trial_iters=10, precision=a.sweep_precision)
-    log("autotune on instance %d (%d cells): best tile %s, %.1f s; %s" %
-        (rep, N[rep], best, time.perf_counter() - t0, [(t[0], round(t[1], 4)) for t in table]))
+    if a.tile_cells and a.tile_threads:
+        best = (a.tile_cells, a.tile_threads)
+        log("tile fixed at %s (--tile-cells, --tile-threads)" % (best,))
+    else:
+        rep = int(np.argmax(N)) if a.sweep_tune == "largest" else int(np.argsort(N)[len(N) // 2])
+        t0 = time.perf_counter()
+        best, table = ss.autotune(cases[rep], trial_iters=a.tune_iters, precision=a.sweep_precision)
+        log("autotune on instance %d (%d cells): best tile %s, %.1f s; %s" %
+            (rep, N[rep], best, time.perf_counter() - t0, [(t[0], round(t[1], 4)) for t in table]))
 
     def solve(precision):
         t0 = time.perf_counter()
