@@ -294,6 +294,40 @@ def test_restart_from_the_reference_sav(tmp_path, name):
     assert _sha(prefix + ".dat") == meta["reference_output_sha256"]["dat"]
 
 
+def test_run_arguments_and_a_nonfinite_state(tmp_path):
+    """A context without a state refuses to run (PSST_BAD_ARG); zero
+    iterations change nothing; a precision other than 32 or 64 is
+    PSST_BAD_ARG; a negative temperature gives NaN, reported as
+    PSST_NAN_DETECTED with a message (the reference throws from its flux
+    there, psstConvectiveFlow.cpp:283-286)."""
+    import ctypes
+    mesh, prefix, _, _ = unpack("m_hllc_o2_gg_mm_lc2_rk4", str(tmp_path))
+    case = ss.Case.open(mesh, prefix)
+    L = ss._lib.lib()
+    m, st = case.mesh(), case.settings()
+    h = ctypes.c_void_p()
+    assert L.psst_ctx_create(ctypes.byref(m), ctypes.byref(st), ctypes.byref(h)) == 0
+    try:
+        done = ctypes.c_int32(-1)
+        assert L.psst_run(h, 5, 0.0, None, ctypes.byref(done)) == 1
+        assert b"no state" in L.psst_last_error(h)
+    finally:
+        L.psst_ctx_destroy(h)
+    with ss.Solver(case) as s:
+        before = s.state()
+        assert s.run(0).shape == (0, case.dim + 2) and s.iters_done == 0
+        assert np.array_equal(s.state(), before)
+        with pytest.raises(ss.PsstError) as ei:
+            s.set_precision(16)
+        assert ei.value.status == 1
+        bad = before.copy()
+        bad[3, : case.n_cells // 2] = -300.0
+        s.set_state(bad)
+        with pytest.raises(ss.PsstError) as ei:
+            s.run(5)
+        assert ei.value.status == 3 and "non-finite" in str(ei.value)
+
+
 def _read_sav(path):
     """(iteration, rows of p T u_i) of a .sav file."""
     with open(path) as f:

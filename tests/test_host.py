@@ -217,6 +217,24 @@ def test_settings_with_a_boundary_table_of_another_size_are_rejected(tmp_path):
     assert case.settings().cfl == 0.5
 
 
+@pytest.mark.parametrize("field,value,message", [("dim", 4, "dim"), ("n_cells", 0, "empty mesh"),
+                                                  ("rk_levels", 9, "rk_levels"), ("flux", 7, "flux"),
+                                                  ("order", 3, "order"), ("limiter", 0, "limiter")])
+def test_context_creation_rejects_bad_arguments(tmp_path, field, value, message):
+    """psst_ctx_create validates the mesh and settings before touching the
+    device: PSST_BAD_ARG and a message, no exception across the ABI."""
+    mesh, prefix, _, _ = unpack("m_hllc_o2_gg_mm_lc2_rk4", str(tmp_path))
+    case = ss.Case.open(mesh, prefix)
+    m, s = case.mesh(), case.settings()
+    setattr(m if field in ("dim", "n_cells") else s, field, value)
+    h = ctypes.c_void_p()
+    L = ss._lib.lib()
+    assert L.psst_ctx_create(ctypes.byref(m), ctypes.byref(s), ctypes.byref(h)) == 1
+    assert not h.value
+    assert message in L.psst_last_error(None).decode()
+    assert L.psst_ctx_create(None, ctypes.byref(s), ctypes.byref(h)) == 1
+
+
 def test_trailing_newline_in_boundary_files_is_accepted(tmp_path):
     """The reference crashes on a trailing newline in .bvp/.btq (quirk Q11);
     the host reader accepts it and reads the same boundary set."""
