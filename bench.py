@@ -59,7 +59,7 @@ def parse():
                     help="profile mode: first copy a buffer of this many MiB (a kernel of known traffic that "
                          "calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     # batched sweep workload
-    ap.add_argument("--workload", default="wedge", choices=("wedge", "wedge3d", "sweep", "warmstart"),
+    ap.add_argument("--workload", default="wedge", choices=("wedge", "wedge3d", "sweep", "warmstart", "refcase"),
                     help="wedge: the 4.9M-cell iteration throughput; sweep: batched solves of many wedges to a "
                          "residual tolerance")
     ap.add_argument("--instances", type=int, default=512)
@@ -217,6 +217,58 @@ def run_sweep(a):
     print(json.dumps(out), flush=True)
 
 
+# The reference's only published timing: the elapsed-time column of its
+# committed residual file, row 3000 (wedge_residual.dat:3000), on its
+# author's PC (CPU not stated; VS2017 Release build).
+REFCASE_PUBLISHED_S = 152.856
+
+
+def run_refcase(a):
+    """The reference's own case: its wedge80_240.msh (19,200 quads) with
+    wedge.inp/.bvp/.btq/.par/.prop (HLL, first order, RK1, time step mode 1 in
+    the reference's order, CFL 2.08, 3000 iterations, a save every 1000), kept
+    under tests/golden/ref_wedge80x240.  psst_case_solve runs it as the
+    reference's main() does and writes the same residual file, whose row-3000
+    elapsed time (from the start of the solve, saves at 1000 and 2000
+    included, in whole milliseconds) is compared with the published one."""
+    import hashlib
+    import lzma
+
+    import supersonicflowsolver_amd as ss
+    src = os.path.join(ROOT, "tests", "golden", "ref_wedge80x240")
+    d = tempfile.mkdtemp(prefix="psst_refcase_")
+    for ext in ("inp", "bvp", "btq", "par", "prop"):
+        shutil.copy(os.path.join(src, "wedge." + ext), d)
+    with lzma.open(os.path.join(src, "wedge80_240.msh.xz")) as f, open(os.path.join(d, "wedge80_240.msh"), "wb") as g:
+        g.write(f.read())
+    prefix = os.path.join(d, "wedge")
+    case = ss.Case.open(os.path.join(d, "wedge80_240.msh"), prefix)
+    print("bench: the reference's own case, %d cells, iterations %d-%d" % (case.n_cells, case.first_iter,
+                                                                           case.last_iter), file=sys.stderr,
+          flush=True)
+    t = time.perf_counter()
+    case.solve()
+    wall = time.perf_counter() - t
+    with open(prefix + "_residual.dat") as f:
+        rows = [l.split() for l in f if l.strip()]
+    t3000 = float(rows[2999][1])
+    with lzma.open(os.path.join(src, "wedge.sav.xz")) as f:
+        ref_sav = hashlib.sha256(f.read().replace(b"\r", b"")).hexdigest()
+    with open(prefix + ".sav", "rb") as f:
+        same_sav = hashlib.sha256(f.read()).hexdigest() == ref_sav
+    shutil.rmtree(d, ignore_errors=True)
+    out = {"metric": "wall time of the reference's own 3000-iteration case (elapsed-time column, row 3000)",
+           "value": round(t3000, 3), "unit": "s", "n_gpus": 1, "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": round(t3000 / REFCASE_PUBLISHED_S, 5), "dtype": "f64",
+           "data": "the reference's own mesh and settings files",
+           "config": {"workload": "wedge80_240.msh (19,200 quads), HLL, first order, RK1, time step mode 1 in the "
+                                  "reference's order, CFL 2.08, 3000 iterations, a save every 1000"},
+           "baseline": {"value": REFCASE_PUBLISHED_S, "unit": "s", "hardware": "the reference author's PC (CPU not "
+                        "stated), VS2017 Release build", "source": "wedge_residual.dat:3000, column 2"},
+           "whole_call_s": round(wall, 3), "sav_identical_to_reference": bool(same_sav)}
+    print(json.dumps(out), flush=True)
+
+
 def alg_bytes(kind, N, G, F, nnz, dim, b, last_stage=False):
     """Algorithmic (unique) HBM bytes of one launch: every array the kernel
     must read or write, touched once.  b = state bytes, geometry 8, index 4."""
@@ -310,6 +362,8 @@ def main():
         return run_sweep(a)
     if a.workload == "warmstart":
         return run_warmstart(a)
+    if a.workload == "refcase":
+        return run_refcase(a)
     import numpy as np
 
     import supersonicflowsolver_amd as ss
@@ -399,11 +453,11 @@ def main():
     dom = max(per_iter, key=per_iter.get)
     kind = {"flux": "flux", "update": "update", "grad_gg": "grad", "grad_lsq": "grad", "bc": "bc",
             "stage": "stage", "stage_last": "stage", "update_levels_block": "update",
-            "update_level": "update"}.get(dom, dom)
+            "update_levels_wg": "update", "update_levels_grid": "update", "update_level": "update"}.get(dom, dom)
     nbytes = alg_bytes(kind, N, G, F, nnz, dim, b, last_stage=(dom == "stage_last"))
     mean_ms = stats[dom][1]
     note = None
-    if dom in ("update_levels_block", "update_level"):
+    if dom in ("update_levels_block", "update_levels_wg", "update_levels_grid", "update_level"):
         # time step mode 1 in the reference's order: each stage's update is one sweep over the cells'
         # dependency levels (one launch, or one per level); rate over a whole stage's sweep
         mean_ms = stats[dom][0] * stats[dom][1] / (a.timing_iters * 4)

@@ -810,66 +810,81 @@ __global__ void __launch_bounds__(kBlock) k_ordered_residual(Geom g, State<T> s,
 // One cell of the ordered sweep from the tables: update_cell with
 // dt_local1 (psstLocalAccTime1::calculate_at, psstLocalTimeCalculation.cpp:
 // 90-131) in the same operation order.  Everything but the earlier
-// neighbours' new state is loaded straight from the position.
+// neighbours' new state is loaded straight from the position
+// (ordered_fetch), so it can be loaded before the previous level is done;
+// ordered_apply loads the neighbour states and updates the cell.
 template <int DIM, typename T>
-__device__ __forceinline__ void ordered_update(const Geom &g, const State<T> &s, const Phys &ph, T alpha,
-                                               int last_stage, const OrderedTables &ot, int p) {
-    constexpr int WS = DIM + 2;
-    const int c = ot.cell[p];
-    const T *Rp = static_cast<const T *>(ot.R);
-    T R[WS], w0[WS], uc[DIM];
-#pragma unroll
-    for (int j = 0; j < WS; ++j) {
-        R[j] = Rp[(size_t)j * g.N + p];
-        w0[j] = s.W0[(size_t)j * g.N + c];
-    }
-#pragma unroll
-    for (int k = 0; k < DIM; ++k) uc[k] = s.q[(size_t)k * g.NT + c];
-    const T Tc = s.q[(size_t)(DIM + 1) * g.NT + c];
-    const T V = T(ot.vol[p]);
+struct OrderedIn {
+    int c;
+    T R[DIM + 2], w0[DIM + 2], uc[DIM], Tc, V;
     int nbw[kOrderedW];
     double fnw[kOrderedW][DIM], faw[kOrderedW];
+};
+
+template <int DIM, typename T>
+__device__ __forceinline__ void ordered_fetch(const Geom &g, const State<T> &s, const OrderedTables &ot, int p,
+                                              OrderedIn<DIM, T> &o) {
+    constexpr int WS = DIM + 2;
+    o.c = ot.cell[p];
+    const T *Rp = static_cast<const T *>(ot.R);
+#pragma unroll
+    for (int j = 0; j < WS; ++j) {
+        o.R[j] = Rp[(size_t)j * g.N + p];
+        o.w0[j] = s.W0[(size_t)j * g.N + o.c];
+    }
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) o.uc[k] = s.q[(size_t)k * g.NT + o.c];
+    o.Tc = s.q[(size_t)(DIM + 1) * g.NT + o.c];
+    o.V = T(ot.vol[p]);
 #pragma unroll
     for (int w = 0; w < kOrderedW; ++w) {
         const bool in = w < ot.W;
-        nbw[w] = in ? ot.nb[(size_t)w * g.N + p] : -1;
-        faw[w] = in ? ot.fa[(size_t)w * g.N + p] : 0.0;
+        o.nbw[w] = in ? ot.nb[(size_t)w * g.N + p] : -1;
+        o.faw[w] = in ? ot.fa[(size_t)w * g.N + p] : 0.0;
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) fnw[w][k] = in ? ot.fn[((size_t)k * ot.W + w) * g.N + p] : 0.0;
+        for (int k = 0; k < DIM; ++k) o.fnw[w][k] = in ? ot.fn[((size_t)k * ot.W + w) * g.N + p] : 0.0;
     }
+}
+
+template <int DIM, typename T>
+__device__ __forceinline__ void ordered_apply(const Geom &g, const State<T> &s, const Phys &ph, T alpha,
+                                              int last_stage, const OrderedIn<DIM, T> &o) {
+    constexpr int WS = DIM + 2;
+    const int c = o.c;
     // neighbour states: the new one for neighbours earlier in the update
     // loop, the stage-start one otherwise (ghosts included)
     T un[kOrderedW][DIM], Tn[kOrderedW];
 #pragma unroll
     for (int w = 0; w < kOrderedW; ++w) {
-        const int e = nbw[w] < 0 ? 0 : nbw[w];
+        const int e = o.nbw[w] < 0 ? 0 : o.nbw[w];
         const int nb = e & (kEarlier - 1);
         const T *src = (e & kEarlier) ? s.qn : s.q;
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) un[w][k] = nbw[w] < 0 ? T(0) : src[(size_t)k * g.NT + nb];
-        Tn[w] = nbw[w] < 0 ? T(0) : src[(size_t)(DIM + 1) * g.NT + nb];
+        for (int k = 0; k < DIM; ++k) un[w][k] = o.nbw[w] < 0 ? T(0) : src[(size_t)k * g.NT + nb];
+        Tn[w] = o.nbw[w] < 0 ? T(0) : src[(size_t)(DIM + 1) * g.NT + nb];
     }
     const T gr = T(ph.gamma) * T(ph.rm);
-    T cc = dsqrt(gr * Tc);
+    T cc = dsqrt(gr * o.Tc);
     T den = T(0), nn = T(0);
 #pragma unroll
     for (int w = 0; w < kOrderedW; ++w) {
-        if (nbw[w] < 0) break;
+        if (o.nbw[w] < 0) break;
         T nv = T(0);
         T cn = (dsqrt(gr * Tn[w]) + cc) * T(0.5);
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
-            T t = T(fnw[w][k]);
-            nv += uc[k] * t;
+            T t = T(o.fnw[w][k]);
+            nv += o.uc[k] * t;
             nn += un[w][k] * t;
         }
         nn = (nn + nv) * T(0.5);
-        den += (fabs(nn) + cn) * T(faw[w]);
+        den += (fabs(nn) + cn) * T(o.faw[w]);
     }
+    const T V = o.V;
     const T dt = fdiv(T(ph.cfl) * V, den);
     T wn[WS];
 #pragma unroll
-    for (int j = 0; j < WS; ++j) wn[j] = w0[j] - fdiv(dt * alpha * R[j], V);
+    for (int j = 0; j < WS; ++j) wn[j] = o.w0[j] - fdiv(dt * alpha * o.R[j], V);
     if (last_stage)
 #pragma unroll
         for (int j = 0; j < WS; ++j) s.W0[(size_t)j * g.N + c] = wn[j];
@@ -887,6 +902,14 @@ __device__ __forceinline__ void ordered_update(const Geom &g, const State<T> &s,
     s.qn[(size_t)(DIM + 2) * g.NT + c] = rho;
 }
 
+template <int DIM, typename T>
+__device__ __forceinline__ void ordered_update(const Geom &g, const State<T> &s, const Phys &ph, T alpha,
+                                               int last_stage, const OrderedTables &ot, int p) {
+    OrderedIn<DIM, T> o;
+    ordered_fetch<DIM, T>(g, s, ot, p, o);
+    ordered_apply<DIM, T>(g, s, ph, alpha, last_stage, o);
+}
+
 // The ordered sweep as one cooperative launch over several workgroups: the
 // positions of a level are spread over the grid and a grid-wide barrier
 // separates the levels.  Every workgroup runs the same number of barriers;
@@ -901,6 +924,50 @@ __global__ void __launch_bounds__(kBlock) k_update_levels_grid(Geom g, State<T> 
         const int b = level_off[l], e = level_off[l + 1];
         for (int p = b + tid; p < e; p += nthr) ordered_update<DIM, T>(g, s, ph, alpha, last_stage, ot, p);
         grid.sync();
+    }
+}
+
+// The ordered sweep in one workgroup, a workgroup barrier between levels
+// (small meshes or narrow levels).  With the residual gathered by the
+// parallel pre-pass, each level is only the time step and update chain.
+template <int DIM, typename T>
+__global__ void __launch_bounds__(1024) k_update_levels_wg(Geom g, State<T> s, Phys ph, T alpha, int last_stage,
+                                                           OrderedTables ot, const int *__restrict__ level_off,
+                                                           int n_levels) {
+    for (int l = 0; l < n_levels; ++l) {
+        const int b = level_off[l], e = level_off[l + 1];
+        for (int p = b + (int)threadIdx.x; p < e; p += (int)blockDim.x)
+            ordered_update<DIM, T>(g, s, ph, alpha, last_stage, ot, p);
+        __syncthreads();
+    }
+}
+
+// Same for levels of at most 256 cells, with each thread's position of the
+// next level fetched as soon as its cell of this level is updated: a
+// workgroup barrier waits for LDS operations only, so those loads stay in
+// flight across it.  Nothing they read is written by the level before
+// (tables, residuals, the stage-start state, the cell's own snapshot).
+template <int DIM, typename T>
+__global__ void __launch_bounds__(256) k_update_levels_wg_prefetch(Geom g, State<T> s, Phys ph, T alpha,
+                                                                   int last_stage, OrderedTables ot,
+                                                                   const int *__restrict__ level_off, int n_levels) {
+    const int tid = (int)threadIdx.x, nthr = (int)blockDim.x;
+    OrderedIn<DIM, T> o;
+    bool have = n_levels > 0 && level_off[0] + tid < level_off[1];
+    if (have) ordered_fetch<DIM, T>(g, s, ot, level_off[0] + tid, o);
+    for (int l = 0; l < n_levels; ++l) {
+        const int b = level_off[l], e = level_off[l + 1];
+        if (have) ordered_apply<DIM, T>(g, s, ph, alpha, last_stage, o);
+        for (int p = b + tid + nthr; p < e; p += nthr) ordered_update<DIM, T>(g, s, ph, alpha, last_stage, ot, p);
+        have = false;
+        if (l + 1 < n_levels) {
+            const int pn = level_off[l + 1] + tid;
+            if (pn < level_off[l + 2]) {
+                ordered_fetch<DIM, T>(g, s, ot, pn, o);
+                have = true;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -2010,15 +2077,27 @@ void run_update(psst_ctx *c, int k, unsigned long long *res_row) {
         // meshes; on large ones it runs on one CU: 938 ms per iteration on the
         // 4.9M-cell wedge (about 5,000 levels of up to 1,280 cells), against
         // 325 ms with one multi-workgroup launch per level.
-        if (c->max_level <= 256 || c->N <= 65536) {
-            launch(c, "update_levels_block", k_update_levels_block<DIM, T>, dim3(1), dim3(1024), c->geom, st, c->ph,
-                   alpha, first, last, res_row, (const int *)c->level_cells, (const int *)c->level_off_dev, nl);
-        } else if (!c->batch && c->ot.cell && coop_sweep_blocks<DIM, T>(c) > 0) {
-            // (a batch captures its iterations into graphs: it keeps the per-level launches)
+        const bool narrow = c->max_level <= 256 || c->N <= 65536;
+        if (!c->batch && c->ot.cell && (narrow || coop_sweep_blocks<DIM, T>(c) > 0)) {
+            // residuals in parallel, then the dependent chain of time steps
+            // and updates level by level: in one workgroup when the levels are
+            // narrow (the reference's own 19,200-cell case: 1.8 ms per
+            // iteration before, with the residual gather inside the chain),
+            // else in one cooperative launch with a grid barrier per level
             launch(c, "ordered_residual", k_ordered_residual<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st,
                    c->ot, res_row);
-            launch_coop(c, "update_levels_grid", k_update_levels_grid<DIM, T>, dim3(c->coop_blocks), dim3(kBlock),
-                        c->geom, st, c->ph, alpha, last, c->ot, (const int *)c->level_off_dev, nl);
+            if (narrow && c->max_level <= 256)
+                launch(c, "update_levels_wg", k_update_levels_wg_prefetch<DIM, T>, dim3(1), dim3(256), c->geom, st,
+                       c->ph, alpha, last, c->ot, (const int *)c->level_off_dev, nl);
+            else if (narrow)
+                launch(c, "update_levels_wg", k_update_levels_wg<DIM, T>, dim3(1), dim3(1024), c->geom, st, c->ph,
+                       alpha, last, c->ot, (const int *)c->level_off_dev, nl);
+            else
+                launch_coop(c, "update_levels_grid", k_update_levels_grid<DIM, T>, dim3(c->coop_blocks),
+                            dim3(kBlock), c->geom, st, c->ph, alpha, last, c->ot, (const int *)c->level_off_dev, nl);
+        } else if (narrow) {
+            launch(c, "update_levels_block", k_update_levels_block<DIM, T>, dim3(1), dim3(1024), c->geom, st, c->ph,
+                   alpha, first, last, res_row, (const int *)c->level_cells, (const int *)c->level_off_dev, nl);
         } else {
             for (int l = 0; l < nl; ++l) {
                 int b = c->level_off[l], n = c->level_off[l + 1] - b;
@@ -2376,10 +2455,11 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
             c->max_level = widest;
             c->level_cells = dev_upload(c.get(), cells.data(), cells.size());
             c->level_off_dev = dev_upload(c.get(), c->level_off.data(), c->level_off.size());
-            // tables of the cooperative sweep (large meshes with wide levels)
+            // tables of the ordered sweep (single contexts; a batch keeps the
+            // per-level launches, which it captures into graphs)
             int W = 0;
             for (int i = 0; i < N; ++i) W = std::max(W, m->list_offsets[i + 1] - m->list_offsets[i]);
-            if (N > 65536 && widest > 256 && W <= kOrderedW) {
+            if (seg.empty() && W <= kOrderedW) {
                 std::vector<int> onb((size_t)W * N, -1);
                 std::vector<double> ofn((size_t)dim * W * N, 0.0), ofa((size_t)W * N, 0.0), ovol((size_t)N);
                 for (int p = 0; p < N; ++p) {
@@ -2576,7 +2656,7 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                            "update_levels_grid", "bc",
                            "stage",        "grad_gg_tile",
                            "grad_lsq_tile", "batch_check",
-                           "ordered_residual"};
+                           "ordered_residual", "update_levels_wg"};
     } catch (const std::exception &e) {
         g_create_error = e.what();
         for (void *p : c->allocs) (void)hipFree(p);
