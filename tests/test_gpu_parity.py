@@ -294,6 +294,27 @@ def test_restart_from_the_reference_sav(tmp_path, name):
     assert _sha(prefix + ".dat") == meta["reference_output_sha256"]["dat"]
 
 
+def test_reference_ordered_sweep_with_wide_levels_on_a_small_mesh(tmp_path):
+    """Time step mode 1 in the reference's order on a 12,800-cell wedge with
+    randomly numbered cells: 12 dependency levels of up to 2,629 cells, so
+    the one-workgroup table sweep makes several passes per level.  Bitwise
+    equal to the oracle."""
+    mesh = str(tmp_path / "w.msh")
+    ss.write_wedge_msh(ss.wedge_params(nx=160, ny=80, jitter=0.1, numbering=2), mesh)
+    prefix = str(tmp_path / "c")
+    ss.write_case_files(prefix, cfl=0.6, time_mode=1, rk_levels=2, flux=3, order=2, grad=1, limiter=2)
+    case = ss.Case.open(mesh, prefix)
+    o = Oracle(mesh, prefix)
+    ref_res = o.run(6)
+    with ss.Solver(case) as s:
+        s.kernel_timing(True)
+        res = s.run(6)
+        assert s.kernel_stats("update_levels_wg")[0] == 12
+        st = s.state()
+    assert _first_mismatch(st, o.state()) is None, _first_mismatch(st, o.state())
+    assert np.array_equal(res, ref_res)
+
+
 def test_run_arguments_and_a_nonfinite_state(tmp_path):
     """A context without a state refuses to run (PSST_BAD_ARG); zero
     iterations change nothing; a precision other than 32 or 64 is
