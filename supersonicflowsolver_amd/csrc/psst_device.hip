@@ -920,9 +920,24 @@ __global__ void __launch_bounds__(kBlock) k_update_levels_grid(Geom g, State<T> 
                                                                int n_levels) {
     cooperative_groups::grid_group grid = cooperative_groups::this_grid();
     const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x), nthr = (int)(gridDim.x * blockDim.x);
+    // each thread's first position of the next level is fetched right after
+    // its cell of this level is updated (nothing it reads is written by this
+    // level), so its loads overlap the stores the grid barrier waits for
+    OrderedIn<DIM, T> o;
+    bool have = n_levels > 0 && level_off[0] + tid < level_off[1];
+    if (have) ordered_fetch<DIM, T>(g, s, ot, level_off[0] + tid, o);
     for (int l = 0; l < n_levels; ++l) {
         const int b = level_off[l], e = level_off[l + 1];
-        for (int p = b + tid; p < e; p += nthr) ordered_update<DIM, T>(g, s, ph, alpha, last_stage, ot, p);
+        if (have) ordered_apply<DIM, T>(g, s, ph, alpha, last_stage, o);
+        for (int p = b + tid + nthr; p < e; p += nthr) ordered_update<DIM, T>(g, s, ph, alpha, last_stage, ot, p);
+        have = false;
+        if (l + 1 < n_levels) {
+            const int pn = level_off[l + 1] + tid;
+            if (pn < level_off[l + 2]) {
+                ordered_fetch<DIM, T>(g, s, ot, pn, o);
+                have = true;
+            }
+        }
         grid.sync();
     }
 }
