@@ -13,6 +13,8 @@ LIB = os.path.join(HERE, "libpsst.so")
 
 HOST_SOURCES = ["mesh_io.cpp", "grid.cpp", "setup.cpp", "output.cpp", "meshgen.cpp", "host_api.cpp", "tiles.cpp"]
 DEVICE_SOURCES = ["psst_device.hip", "mlp.hip"]
+# the solver's launch templates, instantiated once per (dimension, state type)
+INSTANCES = [(2, "double"), (2, "float"), (3, "double"), (3, "float")]
 
 # -ffp-contract=off: every multiply and add is rounded separately, as in the
 # x86-64 reference build (no FMA), which FP64 bitwise parity requires.
@@ -28,21 +30,32 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
+    """Compiles every object (in parallel) and links libpsst.so."""
+    from concurrent.futures import ThreadPoolExecutor
+
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
     cc = hipcc()
+    cmds, objs = [], []
     for src in HOST_SOURCES:
         obj = os.path.join(objdir, src.replace(".cpp", ".o"))
-        cmd = [cc, "-x", "c++", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON
-        _run(cmd, verbose)
+        cmds.append([cc, "-x", "c++", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON)
         objs.append(obj)
     for src in DEVICE_SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = [cc, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + DEVICE
-        _run(cmd, verbose)
+        cmds.append([cc, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + DEVICE)
         objs.append(obj)
+    for dim, t in INSTANCES:
+        obj = os.path.join(objdir, "psst_inst_%dd_%s.o" % (dim, t))
+        cmds.append([cc, "-x", "hip", "-c", os.path.join(CSRC, "psst_inst.hip"), "-o", obj,
+                     "-DPSST_INST_DIM=%d" % dim, "-DPSST_INST_T=%s" % t] + COMMON + DEVICE)
+        objs.append(obj)
+    # the slow device objects first
+    order = sorted(range(len(cmds)), key=lambda i: "-x" in cmds[i] and cmds[i][2] == "hip", reverse=True)
+    jobs = jobs or max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda i: _run(cmds[i], verbose), order))
     tmp = LIB + ".tmp"
     _run([cc, "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950"], verbose)
     os.replace(tmp, LIB)

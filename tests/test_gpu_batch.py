@@ -95,6 +95,29 @@ def test_batch_odd_stage_count_and_continuation(tmp_path):
                 assert np.array_equal(b.state(i), s.state())
 
 
+def test_batch_excluded_instance_keeps_its_state_with_odd_stage_count(tmp_path):
+    """RK3 alternates the state buffers every iteration.  An instance
+    excluded after an odd number of iterations keeps the state it had (not
+    the other buffer's stage intermediate), and included again it continues
+    from there: equal to a lone context run for the same iterations."""
+    cases = _cases(tmp_path, rk_levels=3)[:2]
+    with ss.Batch(cases) as b:
+        b.run(1)
+        b.set_enabled(0, False)
+        b.run(1)
+        with ss.Solver(cases[0]) as s0, ss.Solver(cases[1]) as s1:
+            s0.run(1)
+            s1.run(2)
+            assert np.array_equal(b.state(0), s0.state())
+            assert np.array_equal(b.state(1), s1.state())
+            b.set_enabled(0, True)
+            out = b.run(3, history=True)
+            r0, r1 = s0.run(3), s1.run(3)
+            assert np.array_equal(out["hist"][:, 0], r0) and np.array_equal(out["hist"][:, 1], r1)
+            assert np.array_equal(b.state(0), s0.state())
+            assert np.array_equal(b.state(1), s1.state())
+
+
 def test_batch_precision_switch(tmp_path):
     """FP32 iterations, switch to FP64, more iterations: equal to a lone
     context doing the same switch."""

@@ -347,6 +347,13 @@ def test_run_arguments_and_a_nonfinite_state(tmp_path):
         with pytest.raises(ss.PsstError) as ei:
             s.run(5)
         assert ei.value.status == 3 and "non-finite" in str(ei.value)
+        # the run stops at the first non-finite residual row, also with a
+        # tolerance (no iterations are spent on a failed state)
+        assert s.iters_done == 1
+        s.set_state(bad)
+        with pytest.raises(ss.PsstError) as ei:
+            s.run(300, tol=1e-6)
+        assert ei.value.status == 3 and s.iters_done == 1
 
 
 def _read_sav(path):
