@@ -270,24 +270,23 @@ def run_refcase(a):
 
 
 def alg_bytes(kind, N, G, F, nnz, dim, b, last_stage=False):
-    """Algorithmic (unique) HBM bytes of one launch: every array the kernel
-    must read or write, touched once.  b = state bytes, geometry 8, index 4."""
+    """Algorithmic HBM bytes of one launch, the scope's unique-touch model:
+    b = state bytes, geometry 8, index 4, nv = dim + 2 stored state values
+    (u_i, p, T).  Stage: N (nv b [q read] + nv dim b [gradient read] + nv b
+    [w0 read] + nv b [q write] + (1 + dim) 8 [V, x]) + F ((dim + 1) 8 [n, A] +
+    2 * 4 [c0, c1]).  Gradient pass: N (nv b + nv dim b + 8) + F ((dim + 1) 8
+    + 8 + 8 [beta]).  Arrays a kernel reads beyond these (tile face lists,
+    halo cells, time step face sums) count as traffic, not as algorithmic
+    bytes."""
     nv = dim + 2  # u..., p, T
-    if kind == "flux":  # face-based flux, second order
-        rd = N * nv * b + G * (nv + 1) * b + N * nv * dim * b + N * dim * 8 + F * ((dim + 1) * 8 + 2 * 4)
-        return rd + F * nv * b
-    if kind == "update":
-        rd = F * nv * b + N * 4 + nnz * 4 + F * 4 + N * (dim + 1) * b + N * dim * 8 + N * 8 + N * nv * b
-        wr = N * (nv + 1) * b + (N * nv * b if last_stage else 0)
-        return rd + wr
-    if kind == "stage":  # fused stage: flux + gather + update + ghosts
-        rd = (N * (nv + 1) * b + G * (nv + 1) * b + N * nv * dim * b + N * dim * 8 + F * ((dim + 1) * 8 + 2 * 4)
-              + N * 4 + nnz * 4 + N * dim * 8 + N * 8 + N * nv * b)
-        wr = (N + G) * (nv + 1) * b + (N * nv * b if last_stage else 0)
-        return rd + wr
+    if kind == "stage":
+        return N * (nv * b + nv * dim * b + nv * b + nv * b + (1 + dim) * 8) + F * ((dim + 1) * 8 + 2 * 4)
     if kind == "grad":
-        rd = (N + G) * nv * b + F * 8 + F * (dim + 2) * 8 + N * 8 + N * 4 + nnz * 4
-        return rd + N * nv * dim * b
+        return N * (nv * b + nv * dim * b + 8) + F * ((dim + 1) * 8 + 8 + 8)
+    if kind == "flux":  # unfused face kernel: the stage's face part plus the flux write
+        return N * (nv * b + nv * dim * b + dim * 8) + G * (nv + 1) * b + F * ((dim + 1) * 8 + 2 * 4) + F * nv * b
+    if kind == "update":  # unfused cell kernel: the stage's cell part plus the flux read
+        return F * nv * b + N * (nv * b + nv * b + 8) + N * (nv + 1) * b + (N * nv * b if last_stage else 0)
     if kind == "bc":
         return G * (nv + 1) * b * 2 + G * 12
     raise ValueError(kind)
