@@ -148,6 +148,20 @@ psst_status psst_set_state(psst_ctx *ctx, const double *prim);
  * this context produced); *iters_done reports how many ran.  Synchronous. */
 psst_status psst_run(psst_ctx *ctx, int32_t n_iter, double tol, double *res_hist, int32_t *iters_done);
 
+/* Same as psst_run, allocating nothing: no hipMalloc, hipFree or other
+ * allocating call.  No counterpart in the reference; it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid it).  The context must already exist
+ * (psst_ctx_create allocates); the residual slots, control words and
+ * residual rows of the call live in a fixed workspace of about 44 KB per
+ * GPU (one static device array, held until the process ends), whose address
+ * psst_ctx_create takes once per device.  Limits: n_iter <= 1024 per call
+ * (PSST_BAD_ARG beyond), and kernel timing (psst_kernel_timing) must be off.
+ * Every call on one GPU shares that GPU's workspace: calls that overlap on
+ * one GPU (several host threads, or contexts on several streams) race.
+ * Results are bitwise those of psst_run with the same arguments. */
+psst_status psst_run_noalloc(psst_ctx *ctx, int32_t n_iter, double tol, double *res_hist, int32_t *iters_done);
+
 /* Copies the primitive state [(dim+3)][N+G] to the host. */
 psst_status psst_get_state(psst_ctx *ctx, double *prim);
 

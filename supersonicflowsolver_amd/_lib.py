@@ -61,6 +61,8 @@ _SIGS = {
     "psst_ctx_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(Settings), C.POINTER(C.c_void_p)]),
     "psst_set_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "psst_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    "psst_run_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int32)]),
     "psst_get_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "psst_get_conservative": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "psst_set_residual_reference": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),

@@ -5,6 +5,8 @@
 // psst_inst.hip and only declared here.
 #include "psst_device.hpp"
 
+#include <mutex>
+
 using namespace psst_dev;
 
 PSST_DIM_T_ENTRY_POINTS(extern, 2, double)
@@ -83,6 +85,37 @@ __global__ void __launch_bounds__(64) k_run_check(RunScratch *rs, double tol) {
     }
     if (bad) rs->ctl[2] = 1;
     if (bad || (tol > 0.0 && d <= tol)) rs->ctl[0] = it;
+}
+
+}  // namespace
+
+// Workspace of psst_run_noalloc: one RunScratch per GPU, in a fixed device
+// array (each GPU holds its own copy until the process ends).  In host code
+// `scratch` names a host-side stand-in, never a device address: the device
+// address of each GPU's copy is taken once per device with
+// hipGetSymbolAddress (scratch_address) and only that address reaches
+// kernels and copies.
+constexpr size_t kScratchWords = (sizeof(RunScratch) + sizeof(unsigned long long) - 1) / sizeof(unsigned long long);
+static __device__ __align__(256) unsigned long long scratch[kScratchWords];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+RunScratch *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `dev`, looked up on the first call for
+// that device (with it current) and kept.
+RunScratch *scratch_address(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw HipError("device index out of range for the fixed workspace");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!ptr) throw HipError("hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[dev] = static_cast<RunScratch *>(ptr);
+    }
+    return g_scratch_ptr[dev];
 }
 
 template <typename T>
@@ -426,6 +459,10 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
             }
         }
         hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
+        hip_check(hipGetDevice(&c->device), "hipGetDevice");
+        // the fixed workspace of psst_run_noalloc on this device (its address
+        // is taken here, once per device, with the device current)
+        (void)scratch_address(c->device);
         {
             // a kernel of this library must demonstrably run: a HIP runtime
             // other than the one it was built for (e.g. one bundled with
@@ -737,6 +774,29 @@ psst_status psst_run(psst_ctx *c, int32_t n_iter, double tol, double *res_hist, 
     // rest of a chunk's queued iterations return at once), else of the
     // run rows' capacity
     return run_impl(c, c->run, n_iter, tol, res_hist, iters_done, tol > 0.0 ? 64 : kRunRows);
+}
+
+psst_status psst_run_noalloc(psst_ctx *c, int32_t n_iter, double tol, double *res_hist, int32_t *iters_done) {
+    if (!c) return PSST_BAD_ARG;
+    if (n_iter > kRunRows) {
+        c->err = "psst_run_noalloc: n_iter exceeds the fixed workspace's 1024 residual rows per call";
+        return PSST_BAD_ARG;
+    }
+    if (c->timing) {
+        c->err = "psst_run_noalloc: kernel timing creates events; disable it (psst_kernel_timing(ctx, 0))";
+        return PSST_BAD_ARG;
+    }
+    RunScratch *rs = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        rs = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_ptr[c->device] : nullptr;
+    }
+    if (!rs) {
+        c->err = "psst_run_noalloc: no workspace address for the context's device";
+        return PSST_HIP_ERROR;
+    }
+    // one chunk: the whole call fits the workspace's rows
+    return run_impl(c, rs, n_iter, tol, res_hist, iters_done, kRunRows);
 }
 
 psst_status psst_get_state(psst_ctx *c, double *prim) {

@@ -1760,6 +1760,7 @@ struct BatchState {
 
 struct psst_ctx {
     int dim = 2, N = 0, G = 0, F = 0, NT = 0, nnz_visit = 0, nnz_list = 0;
+    int device = 0;  // HIP device the context was created on
     psst_settings s{};
     std::vector<psst_bc> bc_host;
     psst_dev::Phys ph{};
@@ -1912,13 +1913,26 @@ State<T> state(psst_ctx *c) {
 // ---- dispatch of the templated kernels ----
 template <typename K>
 void set_shm(K kernel, size_t bytes) {
-    static std::map<const void *, size_t> done;  // largest size already allowed per kernel
+    // largest size already allowed per kernel, in a fixed table (no heap:
+    // runs of psst_run_noalloc launch through here); once the table is full
+    // the attribute is simply set again
+    struct Entry {
+        const void *k;
+        size_t bytes;
+    };
+    static Entry done[256];
+    static int used = 0;
     if (bytes <= 64 * 1024) return;
-    auto it = done.find((const void *)kernel);
-    if (it != done.end() && it->second >= bytes) return;
+    for (int i = 0; i < used; ++i)
+        if (done[i].k == (const void *)kernel && done[i].bytes >= bytes) return;
     hip_check(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes),
               "hipFuncSetAttribute");
-    done[(const void *)kernel] = bytes;
+    for (int i = 0; i < used; ++i)
+        if (done[i].k == (const void *)kernel) {
+            done[i].bytes = bytes;
+            return;
+        }
+    if (used < 256) done[used++] = Entry{(const void *)kernel, bytes};
 }
 
 template <typename K, typename... Args>
