@@ -59,7 +59,8 @@ def parse():
                     help="profile mode: first copy a buffer of this many MiB (a kernel of known traffic that "
                          "calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     # batched sweep workload
-    ap.add_argument("--workload", default="wedge", choices=("wedge", "wedge3d", "sweep", "warmstart", "refcase"),
+    ap.add_argument("--workload", default="wedge",
+                    choices=("wedge", "wedge3d", "sweep", "warmstart", "refcase", "fp32field"),
                     help="wedge: the 4.9M-cell iteration throughput; sweep: batched solves of many wedges to a "
                          "residual tolerance")
     ap.add_argument("--instances", type=int, default=512)
@@ -78,6 +79,11 @@ def parse():
                     help="32: FP32 phase, then FP64 to the tolerance; 64: FP64 throughout")
     ap.add_argument("--compare-fp64", type=int, default=0, help="sweep: also solve the batch in FP64 throughout")
     ap.add_argument("--graph", type=int, default=1)
+    # FP32 converged-field workload
+    ap.add_argument("--field-config", default="c1", choices=("c1", "c2", "c3", "c3tet"),
+                    help="fp32field: c1 120x40 wedge, c2 3840x1280 jittered wedge, c3 768x256x64 hexahedra, "
+                         "c3tet the same split into tetrahedra")
+    ap.add_argument("--field-max-iter", type=int, default=60000)
     # learned warm start workload
     ap.add_argument("--train-instances", type=int, default=256)
     ap.add_argument("--train-mesh", default="384x128", help="nx x ny of the training solves")
@@ -214,6 +220,107 @@ def run_sweep(a):
         r64 = solve(64)
         out["fp64_only"] = {"wall_s": round(r64["wall"], 3), "converged": int(np.sum(r64["outcome"] == 0)),
                             "iters": {"median": int(np.median(r64["iters64"])), "max": int(r64["iters64"].max())}}
+    print(json.dumps(out), flush=True)
+
+
+def run_fp32field(a):
+    """Converged-field error of the FP32 mode: the case solved in FP64 to a
+    `tol` residual drop; in FP32 for the same iterations (an FP32 state
+    floors the drop near 1e-4); and in two phases, FP32 to a `switch_tol`
+    drop and then FP64 to `tol` (both phases normalised by the FP64 run's
+    first residual row).  Reports the relative L2 error of every field
+    against the converged FP64 field (velocity components against the norm
+    of the speed), the FP32 floor, and the iterations and wall time of the
+    FP64 and the two-phase solves.  The FP64 run is bitwise the oracle's."""
+    import numpy as np
+
+    import supersonicflowsolver_amd as ss
+
+    def log(msg):
+        print("bench: " + msg, file=sys.stderr, flush=True)
+
+    tmp = tempfile.mkdtemp(prefix="psst_field_")
+    prefix = os.path.join(tmp, "case")
+    cfgs = {"c1": ("2D wedge 120x40 quads, 15 degree ramp, column numbering; M 2.5, HLLC + DM/MinMod + "
+                   "Green-Gauss, RK4, time step mode 2, CFL 0.8", dict(nx=120, ny=40)),
+            "c2": ("2D wedge 3840x1280 quads, jitter 10%% seed 1, RCM numbering; M 2.5, HLLC + DM/MinMod + "
+                   "Green-Gauss, RK4, time step mode 2, CFL 0.8", dict(nx=3840, ny=1280, jitter=0.1, jitter_seed=1,
+                                                                   numbering=1)),
+            "c3": ("3D wedge 768x256x64 hexahedra, 20 degree ramp, z-periodic; M 3, AUSM + DM/Venkatakrishnan + "
+                   "least squares, RK4, time step mode 2, CFL 0.8", dict(nx=768, ny=256, nz=64, depth=1.0,
+                                                                         ramp_deg=20.0, periodic_z=1)),
+            "c3tet": ("3D wedge 768x256x64 hexahedra split into 6 tetrahedra each, 20 degree ramp, z-periodic; M 3, "
+                      "AUSM + DM/Venkatakrishnan + least squares, RK4, time step mode 2, CFL 0.8",
+                      dict(nx=768, ny=256, nz=64, depth=1.0, ramp_deg=20.0, periodic_z=1, tets=1))}
+    desc, wp = cfgs[a.field_config]
+    dim = 3 if "nz" in wp else 2
+    if dim == 3:
+        ss.write_case_files(prefix, dim=3, cfl=0.8, time_mode=2, iterations=1000, rk_levels=4, flux=FLUX_IDS["ausm"],
+                            order=2, grad=2, limiter=4, mach=3.0, p=101325.0, T=300.0)
+    else:
+        ss.write_case_files(prefix, cfl=0.8, time_mode=2, iterations=1000, rk_levels=4, flux=FLUX_IDS["hllc"],
+                            order=2, grad=1, limiter=2, mach=2.5, p=101325.0, T=300.0)
+    case = ss.Case.wedge(ss.wedge_params(**wp), prefix)
+    N = case.n_cells
+    log("%s: %d cells" % (a.field_config, N))
+
+    def solve(precision, n, tol, res0=None):
+        s = ss.Solver(case, precision=precision)
+        if res0 is not None:
+            s.set_residual_reference(res0)
+        t = time.perf_counter()
+        r = s.run(n, tol=tol, allow_no_convergence=True)
+        el = time.perf_counter() - t
+        return s, r[:s.iters_done], el
+
+    s64, r64, t64 = solve(64, a.field_max_iter, a.tol)
+    it64 = s64.iters_done
+    ref = s64.state()
+    res0 = r64[0].copy()
+    s64.close()
+    drop64 = float((r64[-1] / res0).max())
+    log("FP64: %d iterations to a %.2e drop, %.1f s" % (it64, drop64, t64))
+    s32, r32, t32 = solve(32, it64, 0.0, res0)
+    st32 = s32.state()
+    s32.close()
+    d32 = (r32 / res0).max(axis=1)
+    log("FP32: %d iterations, %.1f s, lowest drop %.2e" % (it64, t32, d32.min()))
+    # two phases: FP32 to switch_tol, then FP64 to tol
+    st = ss.Solver(case, precision=32)
+    st.set_residual_reference(res0)
+    t = time.perf_counter()
+    st.run(it64, tol=a.switch_tol, allow_no_convergence=True)
+    n1 = st.iters_done
+    st.set_precision(64)
+    st.run(a.field_max_iter, tol=a.tol, allow_no_convergence=True)
+    n2 = st.iters_done
+    t_two = time.perf_counter() - t
+    st_two = st.state()
+    st.close()
+    log("two phases: %d FP32 + %d FP64 iterations, %.1f s" % (n1, n2, t_two))
+    names = ["u", "v", "w", "p", "T", "rho"] if dim == 3 else ["u", "v", "p", "T", "rho"]
+
+    def rel(st_):
+        speed = np.linalg.norm(ref[:dim, :N])
+        out = {}
+        for k, nm in enumerate(names):
+            scale = speed if k < dim else np.linalg.norm(ref[k, :N])
+            out[nm] = float(np.linalg.norm(st_[k, :N] - ref[k, :N]) / scale)
+        return out
+
+    out = {"metric": "FP32 converged-field error against FP64 (relative L2 per field)",
+           "value": max(rel(st32).values()), "unit": "relative L2 (max over fields)", "n_gpus": 1,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (geometry, norms f64)",
+           "data": "synthetic (generated wedge mesh, freestream initial state)",
+           "config": {"workload": desc, "cells": N, "tol": a.tol, "switch_tol": a.switch_tol},
+           "fp64": {"iterations": it64, "final_drop": drop64, "wall_s": round(t64, 3)},
+           "fp32": {"iterations": it64, "wall_s": round(t32, 3), "lowest_drop": float(d32.min()),
+                    "first_iteration_below_switch_tol": int(np.argmax(d32 <= a.switch_tol)) + 1
+                    if (d32 <= a.switch_tol).any() else None,
+                    "rel_l2_vs_converged_fp64": rel(st32)},
+           "two_phase": {"fp32_iterations": n1, "fp64_iterations": n2, "wall_s": round(t_two, 3),
+                         "rel_l2_vs_converged_fp64": rel(st_two), "speedup_vs_fp64": round(t64 / t_two, 4)}}
+    shutil.rmtree(tmp, ignore_errors=True)
     print(json.dumps(out), flush=True)
 
 
@@ -363,6 +470,8 @@ def main():
         return run_warmstart(a)
     if a.workload == "refcase":
         return run_refcase(a)
+    if a.workload == "fp32field":
+        return run_fp32field(a)
     import numpy as np
 
     import supersonicflowsolver_amd as ss

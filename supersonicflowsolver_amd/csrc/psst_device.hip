@@ -749,9 +749,9 @@ psst_status psst_set_state(psst_ctx *c, const double *prim) {
         }
         c->cur = 0;
         if (c->dim == 2) {
-            c->s.precision == 64 ? run_set_w<2, double>(c) : run_set_w<2, float>(c);
+            c->s.precision == 64 ? run_set_w<2, double>(c, 0, c->N) : run_set_w<2, float>(c, 0, c->N);
         } else {
-            c->s.precision == 64 ? run_set_w<3, double>(c) : run_set_w<3, float>(c);
+            c->s.precision == 64 ? run_set_w<3, double>(c, 0, c->N) : run_set_w<3, float>(c, 0, c->N);
         }
         hip_check(hipStreamSynchronize(c->stream), "set_w");
         c->have_state = true;
@@ -1326,6 +1326,7 @@ psst_status psst_batch_create(int32_t n, const psst_mesh_desc *meshes, const pss
         for (int64_t d = 0; d < NT; ++d) bs->c2d[(size_t)c->d2c[(size_t)d]] = (int32_t)d;
         bs->have_state.assign((size_t)n, 0);
         bs->enabled.assign((size_t)n, 1);
+        bs->w_dirty.assign((size_t)n, 0);
         bs->buf_of.assign((size_t)n, 0);
         const int ws = dim + 2;
         psst_ctx *cx = c;
@@ -1388,7 +1389,7 @@ psst_status psst_batch_set_state(psst_batch *bt, int32_t inst, const double *pri
         hip_check(hipMemcpy(b.conv + inst, &m1, sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
         b.have_state[(size_t)inst] = 1;
         b.buf_of[(size_t)inst] = c->cur;
-        b.w_dirty = true;
+        b.w_dirty[(size_t)inst] = 1;
         c->have_state = true;
         return PSST_OK;
     } catch (const std::exception &e) {
@@ -1469,10 +1470,14 @@ psst_status psst_batch_run(psst_batch *bt, int32_t max_iter, double tol, int32_t
     const int n = b.n, ws = b.ws;
     try {
         batch_gather_states(c);
-        if (b.w_dirty) {
-            if (c->dim == 2) c->s.precision == 64 ? run_set_w<2, double>(c) : run_set_w<2, float>(c);
-            else c->s.precision == 64 ? run_set_w<3, double>(c) : run_set_w<3, float>(c);
-            b.w_dirty = false;
+        // conservative state of the instances given a new state (only theirs:
+        // the others keep the exact w of their last iteration)
+        for (int i = 0; i < n; ++i) {
+            if (!b.w_dirty[(size_t)i]) continue;
+            const int c0 = b.cell_begin[(size_t)i], c1 = b.cell_begin[(size_t)i + 1];
+            if (c->dim == 2) c->s.precision == 64 ? run_set_w<2, double>(c, c0, c1) : run_set_w<2, float>(c, c0, c1);
+            else c->s.precision == 64 ? run_set_w<3, double>(c, c0, c1) : run_set_w<3, float>(c, c0, c1);
+            b.w_dirty[(size_t)i] = 0;
         }
         // Every transfer of the run goes through the pinned host buffers
         // allocated with the batch (h_flags: active [n], active-tile count,

@@ -1081,12 +1081,13 @@ __global__ void __launch_bounds__(kBlock) k_bc(Geom g, T *__restrict__ qn, const
     bc_apply<DIM, T>(g, qn, bcs, ph, i);
 }
 
-// Conservative variables from primitives for the interior cells
+// Conservative variables from primitives for the interior cells [c_begin, c_end)
 // (psstFiniteVolumeGrid.cpp:279-300).
 template <int DIM, typename T>
-__global__ void __launch_bounds__(kBlock) k_set_w(Geom g, const T *__restrict__ q, T *__restrict__ W0, Phys ph) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= g.N) return;
+__global__ void __launch_bounds__(kBlock) k_set_w(Geom g, const T *__restrict__ q, T *__restrict__ W0, Phys ph,
+                                                  int c_begin, int c_end) {
+    const int c = c_begin + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= c_end) return;
     const size_t NT = (size_t)g.NT;
     T rho = q[(DIM + 2) * NT + c];
     T vsq = T(0);
@@ -1744,7 +1745,7 @@ struct BatchState {
     double *h_ring = nullptr; // [H][n][dim+2] residual ring
     double *h_drop = nullptr; // [n]
     int64_t iter_host = 0;
-    bool w_dirty = false;
+    std::vector<char> w_dirty;  // instances whose conservative state must be formed from their primitives
     bool use_graph = true;
     // Tile kernels of a run launch one workgroup per tile of the batch's
     // active-list capacity: all tiles at first, then the smallest power of
@@ -2197,9 +2198,9 @@ void run_iteration(psst_ctx *c, unsigned long long *res_row) {
 }
 
 template <int DIM, typename T>
-void run_set_w(psst_ctx *c) {
-    launch(c, "set_w", k_set_w<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, (const T *)c->q[c->cur],
-           (T *)c->W0, c->ph);
+void run_set_w(psst_ctx *c, int c_begin, int c_end) {
+    launch(c, "set_w", k_set_w<DIM, T>, dim3(blocks(c_end - c_begin)), dim3(kBlock), c->geom, (const T *)c->q[c->cur],
+           (T *)c->W0, c->ph, c_begin, c_end);
 }
 
 // One convective pass with the gradients last computed, gathered per cell
@@ -2218,6 +2219,6 @@ void run_debug_residual(psst_ctx *c, double *out) {
 // psst_device.hip (EXT extern).
 #define PSST_DIM_T_ENTRY_POINTS(EXT, DIM, T)                                                  \
     EXT template void psst_dev::run_iteration<DIM, T>(psst_ctx *, unsigned long long *);     \
-    EXT template void psst_dev::run_set_w<DIM, T>(psst_ctx *);                                \
+    EXT template void psst_dev::run_set_w<DIM, T>(psst_ctx *, int, int);                      \
     EXT template void psst_dev::run_grad<DIM, T>(psst_ctx *);                                 \
     EXT template void psst_dev::run_debug_residual<DIM, T>(psst_ctx *, double *);

@@ -84,19 +84,27 @@ def build_cases(instances, workdir, threads=16, prefix="s", **case_kw):
         return list(ex.map(one, range(len(instances))))
 
 
-def solve_to_tolerance(batch, tol=1e-8, switch_tol=1e-3, chunk=1000, budget=None, diverge=1e3, stall=0.7, log=None):
+def solve_to_tolerance(batch, tol=1e-8, switch_tol=1e-3, chunk=1000, budget=None, diverge=1e3, stall=0.7, log=None,
+                       fallback=None, restart_growth=100.0):
     """Runs a batch until every instance's residual drop reaches `tol`.
 
     If the batch is in FP32, each instance first runs in FP32 until its drop
-    reaches `switch_tol` or stops improving (FP32 face-flux sums floor the
-    drop near 1e-4: an instance whose drop, below 1e-2, fell by less than a
+    reaches `switch_tol` or stops improving (an FP32 state floors the drop
+    near 1e-4: an instance whose drop, below 1e-2, fell by less than a
     factor `stall` over the last `chunk` iterations leaves the FP32 phase);
     then the batch switches to FP64 and runs to `tol`.  An instance is given
     up when its drop exceeds `diverge` (or turns non-finite) or when it has
     used its iteration budget (`budget`, per instance; default unlimited).
+
+    `fallback` ({instance: primitive state}) gives instances a second start:
+    one that turns non-finite, exceeds `diverge`, or whose drop rises above
+    `restart_growth` times its best drop so far is reset, once, to that state
+    in the same batch (same residual reference), and the iterations it spent
+    count towards its total.
+
     Returns per-instance iteration counts of both phases, the final drop,
-    the outcome (0 converged, 1 budget spent, 2 diverged) and the wall time
-    of each phase (seconds)."""
+    the outcome (0 converged, 1 budget spent, 2 diverged), whether it was
+    restarted from its fallback, and the wall time of each phase (seconds)."""
     import time
 
     import numpy as np
@@ -106,7 +114,10 @@ def solve_to_tolerance(batch, tol=1e-8, switch_tol=1e-3, chunk=1000, budget=None
     iters = {32: np.zeros(n, dtype=np.int64), 64: np.zeros(n, dtype=np.int64)}
     outcome = np.full(n, -1, dtype=np.int64)
     drop = np.full(n, np.inf)
+    best = np.full(n, np.inf)
+    restarted = np.zeros(n, dtype=bool)
     walls = {32: 0.0, 64: 0.0}
+    fallback = fallback or {}
 
     def phase(prec, target):
         t0 = time.perf_counter()
@@ -123,7 +134,19 @@ def solve_to_tolerance(batch, tol=1e-8, switch_tol=1e-3, chunk=1000, budget=None
                 d = float(out["drop"][i])
                 drop[i] = d
                 used = iters[32][i] + iters[64][i]
-                if out["status"][i] == 3 or not d < diverge:
+                failed = out["status"][i] == 3 or not d < diverge
+                if i in fallback and not restarted[i] and (failed or d > restart_growth * best[i]):
+                    batch.set_state(i, fallback[i])
+                    restarted[i] = True
+                    best[i] = np.inf
+                    prev.pop(i, None)
+                    if used >= budget[i]:
+                        outcome[i] = 1
+                        batch.set_enabled(i, False)
+                        running.discard(i)
+                    continue
+                best[i] = min(best[i], d)
+                if failed:
                     outcome[i] = 2
                 elif out["status"][i] == 0:
                     if prec == 64 or target <= tol:
@@ -145,5 +168,5 @@ def solve_to_tolerance(batch, tol=1e-8, switch_tol=1e-3, chunk=1000, budget=None
     phase(64, tol)
     for i in range(n):
         batch.set_enabled(i, True)
-    return {"iters32": iters[32], "iters64": iters[64], "drop": drop, "outcome": outcome, "wall32": walls[32],
-            "wall64": walls[64], "wall": walls[32] + walls[64]}
+    return {"iters32": iters[32], "iters64": iters[64], "drop": drop, "outcome": outcome, "restarted": restarted,
+            "wall32": walls[32], "wall64": walls[64], "wall": walls[32] + walls[64]}

@@ -135,24 +135,42 @@ def targets(case, prim, mach):
     return y
 
 
-def warm_state(case, mlp, mach, theta):
-    """Initial primitive state from the network: interior cells predicted,
-    ghosts as the case's own initial state (the supersonic inlet's ghosts
-    hold the fixed inflow; the others are rewritten by the boundary
-    conditions after the first stage).  Density and pressure are kept above
-    5% of the freestream."""
+def warm_state(case, mlp, mach, theta, ramp_start=0.5):
+    """Initial primitive state from the network, projected onto states the
+    steady flow can take:
+
+    * ghosts are the case's own initial state (the supersonic inlet's ghosts
+      hold the exact inflow; the others are rewritten by the boundary
+      conditions after the first stage);
+    * cells upstream of the ramp (x < ramp_start) are exactly the
+      freestream: with a supersonic inflow nothing reaches them;
+    * density and pressure stay within [0.3, 40] and [0.3, 60] times the
+      freestream (two oblique shocks and the expansion at the ramp's end stay
+      inside these for Mach 1.5-4 and ramps up to 25 degrees), the speed at
+      most 1.05 times the inflow speed, and the temperature follows from
+      p / (rho rm), so it is positive."""
     prim = case.initial_state()
     s = case.settings()
     rho_i, u_i, p_i = freestream(case, mach)
-    y = mlp.forward(features(centres(case), mach, theta)).astype(np.float64)
+    xy = centres(case)
+    y = mlp.forward(features(xy, mach, theta)).astype(np.float64)
     N = case.n_cells
-    rho = np.maximum((y[:, 0] + 1.0) * rho_i, 0.05 * rho_i)
-    p = np.maximum((y[:, 3] + 1.0) * p_i, 0.05 * p_i)
-    prim[0, :N] = (y[:, 1] + 1.0) * u_i
-    prim[1, :N] = y[:, 2] * u_i
+    rho = np.clip((y[:, 0] + 1.0) * rho_i, 0.3 * rho_i, 40.0 * rho_i)
+    p = np.clip((y[:, 3] + 1.0) * p_i, 0.3 * p_i, 60.0 * p_i)
+    u = (y[:, 1] + 1.0) * u_i
+    v = y[:, 2] * u_i
+    speed = np.hypot(u, v)
+    scale = np.minimum(1.0, 1.05 * u_i / np.maximum(speed, 1e-300))
+    u, v = u * scale, v * scale
+    up = xy[:, 0] < ramp_start
+    rho[up], p[up], u[up], v[up] = rho_i, p_i, u_i, 0.0
+    prim[0, :N] = u
+    prim[1, :N] = v
     prim[2, :N] = p
     prim[3, :N] = p / (rho * s.rm)
     prim[4, :N] = rho
+    # the freestream cells are the case's own values, bit for bit
+    prim[:, :N][:, up] = case.initial_state()[:, :N][:, up]
     return prim
 
 
@@ -184,7 +202,12 @@ def converged_fields(instances, workdir, nx, ny, tol=1e-8, budget_factor=20, til
 def ab_test(mlp, instances, workdir, nx, ny, tol=1e-8, budget_factor=20, tile=None, log=None):
     """Time to a `tol` residual drop from the freestream (A) and from the
     network's prediction (B), both batched, for the same instances; the drop
-    of both is normalised by the freestream start's first residual."""
+    of both is normalised by the freestream start's first residual.  A warm
+    start that goes non-finite or diverges is reset to the freestream in the
+    same batch (the plain solver is the fallback) and the iterations it had
+    spent count against the warm arm.  Returns both arms' outcomes, the
+    surrogate's error against the plain arm's converged field, and the
+    difference between the two arms' converged fields."""
     inst = [dict(d, nx=nx, ny=ny) for d in instances]
     cases = build_cases(inst, workdir, prefix="e", cfl=0.8, time_mode=2, rk_levels=4, flux=3, order=2, grad=1,
                         limiter=2)
@@ -210,17 +233,23 @@ def ab_test(mlp, instances, workdir, nx, ny, tol=1e-8, budget_factor=20, tile=No
         b.set_residual_reference(i, res0[i])
     if log:
         log("warm starts (network prediction, %.2f s)" % t_pred)
-    rb = solve_to_tolerance(b, tol=tol, budget=budget, log=log)
+    rb = solve_to_tolerance(b, tol=tol, budget=budget, log=log, fallback={i: fresh[i] for i in range(n)})
+    conv_b = [b.state(i) for i in range(n)]
     b.close()
     # surrogate error against the converged field: RMS over the interior
     # cells of (u, v, p, rho) differences, each scaled by its freestream
-    # magnitude (v by the inflow speed)
+    # magnitude (v by the inflow speed); the same measure between the two
+    # arms' converged fields
     err = np.zeros(n)
+    field_diff = np.full(n, np.nan)
     for i in range(n):
         N = cases[i].n_cells
         rho_i, u_i, p_i = freestream(cases[i], inst[i]["mach"])
         scale = np.array([u_i, u_i, p_i, rho_i])[:, None]
         d = (warm[i][[0, 1, 2, 4], :N] - conv_a[i][[0, 1, 2, 4], :N]) / scale
         err[i] = np.sqrt(np.mean(d * d))
-    return {"plain": ra, "warm": rb, "surrogate_rel_l2": err, "predict_s": t_pred,
+        if ra["outcome"][i] == 0 and rb["outcome"][i] == 0:
+            d = (conv_b[i][[0, 1, 2, 4], :N] - conv_a[i][[0, 1, 2, 4], :N]) / scale
+            field_diff[i] = np.sqrt(np.mean(d * d))
+    return {"plain": ra, "warm": rb, "surrogate_rel_l2": err, "converged_field_diff": field_diff, "predict_s": t_pred,
             "cells": np.array([c.n_cells for c in cases])}
