@@ -79,11 +79,16 @@ def parse():
                     help="32: FP32 phase, then FP64 to the tolerance; 64: FP64 throughout")
     ap.add_argument("--compare-fp64", type=int, default=0, help="sweep: also solve the batch in FP64 throughout")
     ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--budget-nx", type=int, default=20, help="sweep: iteration budget per instance, this x nx")
+    ap.add_argument("--sweep-table", default="", help="sweep: write a per-instance CSV table to this path")
     # FP32 converged-field workload
     ap.add_argument("--field-config", default="c1", choices=("c1", "c2", "c3", "c3tet"),
                     help="fp32field: c1 120x40 wedge, c2 3840x1280 jittered wedge, c3 768x256x64 hexahedra, "
                          "c3tet the same split into tetrahedra")
     ap.add_argument("--field-max-iter", type=int, default=60000)
+    ap.add_argument("--field-mesh", default="", help="fp32field, 2D configs: nx x ny instead of the config's mesh")
+    ap.add_argument("--switch-tols", default="", help="fp32field: comma-separated FP32 -> FP64 switch drops "
+                                                      "(default: --switch-tol)")
     # learned warm start workload
     ap.add_argument("--train-instances", type=int, default=256)
     ap.add_argument("--train-mesh", default="384x128", help="nx x ny of the training solves")
@@ -167,7 +172,8 @@ def run_sweep(a):
     import numpy as np
 
     import supersonicflowsolver_amd as ss
-    from supersonicflowsolver_amd.sweep import build_cases, draw_instances, solve_to_tolerance
+    from supersonicflowsolver_amd.sweep import (build_cases, draw_instances, reflection_regime, solve_to_tolerance,
+                                                theta_max)
 
     def log(msg):
         print("bench: " + msg, file=sys.stderr, flush=True)
@@ -194,7 +200,7 @@ def run_sweep(a):
         b = ss.Batch(cases, precision=precision, tile_cells=best[0], tile_threads=best[1])
         b.set_graph(a.graph)
         log("batch context (precision %d): %s, create %.1f s" % (precision, b.info(), time.perf_counter() - t0))
-        budget = [min(a.max_iter, 20 * d["nx"]) for d in inst]
+        budget = [min(a.max_iter, a.budget_nx * d["nx"]) for d in inst]
         r = solve_to_tolerance(b, tol=a.tol, switch_tol=a.switch_tol, budget=budget, log=log)
         b.close()
         return r
@@ -216,6 +222,23 @@ def run_sweep(a):
            "wall_fp32_phase_s": round(r["wall32"], 3), "wall_fp64_phase_s": round(r["wall64"], 3),
            "iters_fp32": {"median": int(np.median(r["iters32"])), "max": int(r["iters32"].max())},
            "iters_fp64": {"median": int(np.median(r["iters64"])), "max": int(r["iters64"].max())}}
+    # outcome against how each instance's ramp shock reflects from the top
+    # wall (regular reflection, Mach reflection, or a detached ramp shock)
+    tmax = np.array([theta_max(d["mach"]) for d in inst])
+    regime = np.array([reflection_regime(d["mach"], d["theta"]) for d in inst])
+    out["outcomes_by_shock"] = {
+        kind: {"instances": int(sel.sum()), "converged": int(np.sum(sel & (r["outcome"] == 0))),
+               "budget_spent": int(np.sum(sel & (r["outcome"] == 1))), "diverged": int(np.sum(sel & (r["outcome"] == 2)))}
+        for kind, sel in (("regular reflection", regime == "regular"), ("Mach reflection", regime == "mach"),
+                          ("detached ramp shock", regime == "detached"))}
+    out["config"]["budget"] = "%d x nx iterations" % a.budget_nx
+    if a.sweep_table:
+        with open(a.sweep_table, "w") as f:
+            f.write("instance,cells,nx,ny,mach,theta_deg,theta_max_deg,regime,outcome,iterations,final_drop\n")
+            for i, d in enumerate(inst):
+                f.write("%d,%d,%d,%d,%.4f,%.3f,%.3f,%s,%s,%d,%.3e\n" % (
+                    i, N[i], d["nx"], d["ny"], d["mach"], d["theta"], tmax[i], regime[i],
+                    ("converged", "budget", "diverged")[int(r["outcome"][i])], it_tot[i], r["drop"][i]))
     if a.compare_fp64 and a.sweep_precision == 32:
         r64 = solve(64)
         out["fp64_only"] = {"wall_s": round(r64["wall"], 3), "converged": int(np.sum(r64["outcome"] == 0)),
@@ -254,6 +277,10 @@ def run_fp32field(a):
                       dict(nx=768, ny=256, nz=64, depth=1.0, ramp_deg=20.0, periodic_z=1, tets=1))}
     desc, wp = cfgs[a.field_config]
     dim = 3 if "nz" in wp else 2
+    if a.field_mesh and dim == 2:
+        nx, ny = (int(v) for v in a.field_mesh.split("x"))
+        desc = desc.replace("%dx%d" % (wp["nx"], wp["ny"]), "%dx%d" % (nx, ny))
+        wp = dict(wp, nx=nx, ny=ny)
     if dim == 3:
         ss.write_case_files(prefix, dim=3, cfl=0.8, time_mode=2, iterations=1000, rk_levels=4, flux=FLUX_IDS["ausm"],
                             order=2, grad=2, limiter=4, mach=3.0, p=101325.0, T=300.0)
@@ -285,19 +312,27 @@ def run_fp32field(a):
     s32.close()
     d32 = (r32 / res0).max(axis=1)
     log("FP32: %d iterations, %.1f s, lowest drop %.2e" % (it64, t32, d32.min()))
-    # two phases: FP32 to switch_tol, then FP64 to tol
-    st = ss.Solver(case, precision=32)
-    st.set_residual_reference(res0)
-    t = time.perf_counter()
-    st.run(it64, tol=a.switch_tol, allow_no_convergence=True)
-    n1 = st.iters_done
-    st.set_precision(64)
-    st.run(a.field_max_iter, tol=a.tol, allow_no_convergence=True)
-    n2 = st.iters_done
-    t_two = time.perf_counter() - t
-    st_two = st.state()
-    st.close()
-    log("two phases: %d FP32 + %d FP64 iterations, %.1f s" % (n1, n2, t_two))
+    # two phases: FP32 to a switch drop, then FP64 to tol
+    two = []
+    for sw in ([float(x) for x in a.switch_tols.split(",")] if a.switch_tols else [a.switch_tol]):
+        st = ss.Solver(case, precision=32)
+        st.set_residual_reference(res0)
+        t = time.perf_counter()
+        st.run(it64, tol=sw, allow_no_convergence=True)
+        n1 = st.iters_done
+        st.set_precision(64)
+        st.run(a.field_max_iter, tol=a.tol, allow_no_convergence=True)
+        n2 = st.iters_done
+        t_two = time.perf_counter() - t
+        two.append((sw, n1, n2, t_two, st.state()))
+        st.close()
+        log("two phases, switch at %.0e: %d FP32 + %d FP64 iterations, %.1f s" % (sw, n1, n2, t_two))
+
+    def first_below(d, x):
+        return int(np.argmax(d <= x)) + 1 if (d <= x).any() else None
+
+    d64 = (r64 / res0).max(axis=1)
+    levels = (1e-1, 1e-2, 1e-3, 1e-4, 1e-6, 1e-8)
     names = ["u", "v", "w", "p", "T", "rho"] if dim == 3 else ["u", "v", "p", "T", "rho"]
 
     def rel(st_):
@@ -313,13 +348,14 @@ def run_fp32field(a):
            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (geometry, norms f64)",
            "data": "synthetic (generated wedge mesh, freestream initial state)",
            "config": {"workload": desc, "cells": N, "tol": a.tol, "switch_tol": a.switch_tol},
-           "fp64": {"iterations": it64, "final_drop": drop64, "wall_s": round(t64, 3)},
+           "fp64": {"iterations": it64, "final_drop": drop64, "wall_s": round(t64, 3),
+                    "first_iteration_below": {"%g" % x: first_below(d64, x) for x in levels}},
            "fp32": {"iterations": it64, "wall_s": round(t32, 3), "lowest_drop": float(d32.min()),
-                    "first_iteration_below_switch_tol": int(np.argmax(d32 <= a.switch_tol)) + 1
-                    if (d32 <= a.switch_tol).any() else None,
+                    "first_iteration_below": {"%g" % x: first_below(d32, x) for x in levels},
                     "rel_l2_vs_converged_fp64": rel(st32)},
-           "two_phase": {"fp32_iterations": n1, "fp64_iterations": n2, "wall_s": round(t_two, 3),
-                         "rel_l2_vs_converged_fp64": rel(st_two), "speedup_vs_fp64": round(t64 / t_two, 4)}}
+           "two_phase": [{"switch_drop": sw, "fp32_iterations": n1, "fp64_iterations": n2, "wall_s": round(tt, 3),
+                          "rel_l2_vs_converged_fp64": rel(stt), "speedup_vs_fp64": round(t64 / tt, 4)}
+                         for sw, n1, n2, tt, stt in two]}
     shutil.rmtree(tmp, ignore_errors=True)
     print(json.dumps(out), flush=True)
 

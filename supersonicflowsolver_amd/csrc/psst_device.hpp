@@ -1292,9 +1292,13 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, con
     for (int j = 0; j < WS; ++j) rmax[j] = 0;
     for (int i = threadIdx.x; i < nc; i += blockDim.x) {
         const int c = cb + i;
-        T R[WS];
+        // the residual is summed, and the update formed, in FP64 whatever the
+        // state type: in FP32 mode the face fluxes and the state are FP32,
+        // their sum and w0 - dt alpha R / V are not rounded to it before the
+        // new state is (in FP64 mode this is the reference's own arithmetic)
+        double R[WS];
 #pragma unroll
-        for (int j = 0; j < WS; ++j) R[j] = T(0);
+        for (int j = 0; j < WS; ++j) R[j] = 0.0;
         // The cell's independent loads (iteration-start state, volume, face
         // sums, visit list) are issued together before the gather: measured
         // 3% (FP64) and 6% (FP32) faster than loading each where it is used.
@@ -1373,8 +1377,7 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, con
         T w[WS];
 #pragma unroll
         for (int j = 0; j < WS; ++j) {
-            T w0 = w0v[j];
-            w[j] = w0 - fdiv(dt * alpha * R[j], V);
+            w[j] = T(double(w0v[j]) - double(dt) * double(alpha) * R[j] / double(V));
             unsigned long long ab = abs_bits(R[j]);
             rmax[j] = rmax[j] > ab ? rmax[j] : ab;
         }

@@ -62,6 +62,54 @@ def draw_instances(n, seed, cells=(20000, 400000), mach=(1.5, 4.0), theta=(5.0, 
     return out
 
 
+def theta_max(mach, gamma=1.4, n=4000):
+    """Largest deflection (degrees) an attached oblique shock turns a flow of
+    Mach number `mach` through: the maximum over the shock angle beta of the
+    theta-beta-Mach relation tan(theta) = 2 cot(beta) (M^2 sin^2(beta) - 1) /
+    (M^2 (gamma + cos(2 beta)) + 2).  A ramp steeper than this detaches the
+    shock."""
+    mu = math.asin(1.0 / mach)
+    best = 0.0
+    for k in range(1, n):
+        b = mu + (math.pi / 2 - mu) * k / n
+        t = math.atan(2.0 / math.tan(b) * (mach * mach * math.sin(b) ** 2 - 1.0) /
+                      (mach * mach * (gamma + math.cos(2.0 * b)) + 2.0))
+        best = max(best, t)
+    return math.degrees(best)
+
+
+def oblique_shock(mach, theta_deg, gamma=1.4, n=4000):
+    """Weak oblique shock of a flow at Mach `mach` turned through
+    `theta_deg`: (shock angle in degrees, Mach number behind it), or None
+    when the turn exceeds theta_max (detached shock).  The smallest shock
+    angle of the theta-beta-Mach relation that reaches the turn, then the
+    normal-shock relation for the normal Mach component."""
+    th = math.radians(theta_deg)
+    mu = math.asin(1.0 / mach)
+    for k in range(1, n):
+        b = mu + (math.pi / 2 - mu) * k / n
+        t = math.atan(2.0 / math.tan(b) * (mach * mach * math.sin(b) ** 2 - 1.0) /
+                      (mach * mach * (gamma + math.cos(2.0 * b)) + 2.0))
+        if t >= th:
+            mn1 = mach * math.sin(b)
+            mn2 = math.sqrt((1.0 + 0.5 * (gamma - 1.0) * mn1 * mn1) / (gamma * mn1 * mn1 - 0.5 * (gamma - 1.0)))
+            return math.degrees(b), mn2 / math.sin(b - th)
+    return None
+
+
+def reflection_regime(mach, theta_deg, gamma=1.4):
+    """How the ramp's shock meets the channel's top wall: "regular" when the
+    flow behind it is supersonic and can be turned back through theta by a
+    reflected oblique shock (theta <= theta_max(M2)), "mach" when it cannot
+    (a Mach reflection, with a subsonic region behind its stem), "detached"
+    when the ramp itself is steeper than theta_max(mach)."""
+    sh = oblique_shock(mach, theta_deg, gamma)
+    if sh is None:
+        return "detached"
+    m2 = sh[1]
+    return "regular" if m2 > 1.0 and theta_deg <= theta_max(m2, gamma) else "mach"
+
+
 def make_case(inst, workdir, name, jitter=0.1, jitter_seed=1, numbering=1, **case_kw):
     """Writes the settings files of one instance and opens it as a Case on a
     generated wedge mesh (ramp angle inst['theta'], inflow Mach inst['mach'])."""

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import supersonicflowsolver_amd as ss
+from oracle.oracle import Oracle
 from supersonicflowsolver_amd.sweep import draw_instances, make_case
 
 pytestmark = pytest.mark.gpu
@@ -178,3 +179,53 @@ def test_run_with_tolerance_ends_on_the_stopping_iteration(tmp_path):
         r2 = s2.run(done)
         assert np.array_equal(r1[:done], r2)
         assert np.array_equal(s1.state(), s2.state())
+
+
+def _oracle(case, prefix):
+    raw = case.raw()
+    raw["dim"] = case.dim
+    return Oracle(None, prefix, raw=raw)
+
+
+def test_batch_fp64_instances_match_the_oracle(tmp_path):
+    """Each FP64 instance of a batch is bitwise the CPU oracle's solve of
+    the same case (residual rows and final state), for uneven meshes in RCM
+    and random numbering."""
+    cases = _cases(tmp_path)[:3]
+    with ss.Batch(cases) as b:
+        out = b.run(60, history=True)
+        for i, c in enumerate(cases):
+            o = _oracle(c, str(tmp_path / ("i%d" % i)))
+            res = o.run(60)
+            print("instance %d: %d cells, rows equal %s, state equal %s" %
+                  (i, c.n_cells, np.array_equal(out["hist"][:, i], res), np.array_equal(b.state(i), o.state())),
+                  flush=True)
+            assert np.array_equal(out["hist"][:, i], res)
+            assert np.array_equal(b.state(i), o.state())
+
+
+def test_batch_fp32_converged_field_against_the_fp64_oracle(tmp_path):
+    """An FP32 instance of a batch, run well past its residual floor, against
+    the oracle's FP64 solve converged to a 1e-8 drop on the 120x40 wedge: the
+    converged fields agree to a few 1e-6 (relative L2 per field, velocity
+    against the speed), and the FP32 drop floors between 1e-5 and 1e-3."""
+    wedge = make_case({"nx": 120, "ny": 40, "mach": 2.5, "theta": 15.0}, str(tmp_path), "c1", jitter=0.0,
+                      numbering=0)
+    other = _cases(tmp_path)[1]
+    with ss.Batch([wedge, other], precision=32) as b:
+        out = b.run(3500, history=True)
+        st32 = b.state(0)
+    o = _oracle(wedge, str(tmp_path / "c1"))
+    res64 = o.run(3500)
+    ref = o.state()
+    drop64 = (res64[-1] / res64[0]).max()
+    d32 = (out["hist"][:, 0] / out["hist"][0, 0]).max(axis=1)
+    N = wedge.n_cells
+    speed = np.linalg.norm(ref[:2, :N])
+    err = {nm: float(np.linalg.norm(st32[k, :N] - ref[k, :N]) / (speed if k < 2 else np.linalg.norm(ref[k, :N])))
+           for k, nm in enumerate(("u", "v", "p", "T", "rho"))}
+    print("oracle FP64 drop %.2e after 3500 iterations; FP32 lowest drop %.2e; FP32 field error %s" %
+          (drop64, d32.min(), {k: "%.2e" % v for k, v in err.items()}), flush=True)
+    assert drop64 < 1e-8
+    assert 1e-5 < d32.min() < 1e-3
+    assert max(err.values()) < 5e-6

@@ -60,3 +60,25 @@ def test_oracle_solves_the_sod_shock_tube(tmp_path, flux):
     assert e["rho"] < 3e-3 and e["p"] < 2e-3 and e["u"] < 3e-3, e
     assert abs(e["p_star"]) < 1e-3, e
     assert e["v_max"] < 1e-6, e
+
+
+@pytest.mark.parametrize("mach,theta,beta,m2", [(2.0, 10.0, 39.31, 1.641), (2.5, 15.0, 36.94, 1.874),
+                                                (3.0, 20.0, 37.76, 1.994)])
+def test_sweep_shock_classifier_known_values(mach, theta, beta, m2):
+    """The sweep module's oblique-shock solution (used to classify sweep
+    instances) against the NACA Report 1135 chart values and the test
+    module's bisection solver; theta_max against the chart's detachment
+    angles; the reflection regime at the three corners of its map."""
+    from supersonicflowsolver_amd.sweep import oblique_shock as sweep_shock
+    from supersonicflowsolver_amd.sweep import reflection_regime, theta_max
+    b, M2 = sweep_shock(mach, theta)
+    assert b == pytest.approx(beta, abs=0.02)
+    assert b == pytest.approx(oblique_shock(mach, theta, 1.4)[0], abs=0.02)
+    assert M2 == pytest.approx(m2, abs=2e-3)
+    # detachment angles of the charts (gamma = 1.4)
+    for m, tmax in ((1.5, 12.11), (2.0, 22.97), (3.0, 34.07)):
+        assert theta_max(m) == pytest.approx(tmax, abs=0.02)
+    assert sweep_shock(1.5, 13.0) is None and reflection_regime(1.5, 13.0) == "detached"
+    assert reflection_regime(3.0, 10.0) == "regular"
+    # M 2, 20 degrees: M2 = 1.21 cannot turn back 20 degrees (theta_max(1.21) = 3.5 degrees)
+    assert reflection_regime(2.0, 20.0) == "mach"
