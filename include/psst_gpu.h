@@ -297,6 +297,30 @@ psst_status psst_case_raw_counts(const psst_case *c, int64_t *counts);
 psst_status psst_case_raw(const psst_case *c, double *xyz, int8_t *face_nv, int64_t *face_v, int64_t *face_c,
                           int64_t *face_zone, int64_t *cell_zone, int64_t *zone_id, int64_t *zone_range,
                           int64_t *zone_bc, char *zone_names, int32_t names_len);
+/* One zone of a mesh flattened by the caller: its Fluent id and type
+ * (1 fluid cells, 2 interior faces, a boundary type, 31/32 parents), its
+ * element count (cells for types 1 and 32, faces otherwise) and the name the
+ * .bvp/.btq files use (psstFaceStorage / psstVolumeStorage,
+ * psstFiniteVolumeGridTools.h:336-360). */
+typedef struct psst_zone {
+    int64_t id;
+    int32_t type;
+    int32_t count;
+    const char *name;
+} psst_zone;
+
+/* Reads the five settings files `<prefix>.inp/.prop/.par/.bvp/.btq` (and
+ * `.sav` when .inp asks for a restart) over a mesh the caller flattened
+ * itself, e.g. from the reference's own psstGridFV with
+ * include/psst_ref_adapter.hpp: the settings and initial state that
+ * psstGridFV::solver_parameters derives (psstFiniteVolumeGrid.cpp:36-194,
+ * :403-717).  zones lists every zone in ascending id; mesh->ghost_bc indexes
+ * it.  The case has no Fluent file behind it, so the writers and
+ * psst_case_solve are PSST_UNSUPPORTED for it (the reference's own
+ * save_data writes the files). */
+psst_status psst_case_from_mesh(const psst_mesh_desc *mesh, int32_t n_zones, const psst_zone *zones,
+                                const char *case_prefix, psst_case **out);
+
 /* Writes the reference's `.sav` checkpoint for a primitive state. */
 psst_status psst_case_write_sav(const psst_case *c, const double *prim, int64_t iter_value, const char *path);
 /* Writes the reference's `.dat` field file for a primitive state [(dim+3)][N+G]:

@@ -45,6 +45,10 @@ class Settings(C.Structure):
     ]
 
 
+class Zone(C.Structure):
+    _fields_ = [("id", C.c_int64), ("type", C.c_int32), ("count", C.c_int32), ("name", C.c_char_p)]
+
+
 class WedgeParams(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
@@ -78,6 +82,8 @@ _SIGS = {
     "psst_wedge_write_msh": (C.c_int, [C.POINTER(WedgeParams), C.c_char_p]),
     "psst_case_open_wedge": (C.c_int, [C.POINTER(WedgeParams), C.c_char_p, C.POINTER(C.c_void_p)]),
     "psst_case_mesh": (C.c_int, [C.c_void_p, C.POINTER(MeshDesc)]),
+    "psst_case_from_mesh": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, C.POINTER(Zone), C.c_char_p,
+                                      C.POINTER(C.c_void_p)]),
     "psst_case_settings": (C.c_int, [C.c_void_p, C.POINTER(Settings)]),
     "psst_case_set_settings": (C.c_int, [C.c_void_p, C.POINTER(Settings)]),
     "psst_case_initial_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),

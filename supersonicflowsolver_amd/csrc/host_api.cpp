@@ -6,6 +6,7 @@
 // (psstFiniteVolumeGrid.h:52-85): settings_file_name(), solver_parameters(),
 // explicit_solve(), save_data().  Failures surface as std::runtime_error, as
 // the reference's own exceptions do.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -187,6 +188,89 @@ psst_status psst_case_open_wedge(const psst_wedge_params *p, const char *case_pr
     }
 }
 
+psst_status psst_case_from_mesh(const psst_mesh_desc *m, int32_t n_zones, const psst_zone *zones,
+                                const char *case_prefix, psst_case **out) {
+    if (!m || !zones || n_zones <= 0 || !case_prefix || !out) return PSST_BAD_ARG;
+    try {
+        auto pc = std::make_unique<psst_case>();
+        psst::Grid &g = pc->c.grid;
+        const int dim = m->dim;
+        if (dim != 2 && dim != 3) throw std::invalid_argument("psst_case_from_mesh: dim must be 2 or 3");
+        const int32_t N = m->n_cells, G = m->n_ghosts, F = m->n_faces, NT = N + G;
+        if (N <= 0 || G < 0 || F <= 0) throw std::invalid_argument("psst_case_from_mesh: empty mesh");
+        g.dim = dim;
+        g.N = N;
+        g.G = G;
+        g.F = F;
+        g.vol.assign(m->cell_volume, m->cell_volume + NT);
+        g.cx.assign(3 * (size_t)NT, 0.0);
+        g.fn.assign(3 * (size_t)F, 0.0);
+        for (int k = 0; k < dim; ++k) {
+            std::copy(m->cell_centre + (size_t)k * NT, m->cell_centre + (size_t)(k + 1) * NT, g.cx.begin() + (size_t)k * NT);
+            std::copy(m->face_normal + (size_t)k * F, m->face_normal + (size_t)(k + 1) * F, g.fn.begin() + (size_t)k * F);
+        }
+        g.fa.assign(m->face_area, m->face_area + F);
+        g.fc.assign(3 * (size_t)F, 0.0);  // face centres are not needed past the geometry the caller computed
+        g.fcell.assign(m->face_cells, m->face_cells + 2 * (size_t)F);
+        g.f0.assign(m->face_cells, m->face_cells + F);
+        g.f1.assign(m->face_cells + F, m->face_cells + 2 * (size_t)F);
+        g.beta.assign(m->face_beta, m->face_beta + F);
+        g.visit_off.assign(m->visit_offsets, m->visit_offsets + N + 1);
+        g.visit_face.assign(m->visit_faces, m->visit_faces + m->visit_offsets[N]);
+        g.list_off.assign(m->list_offsets, m->list_offsets + N + 1);
+        g.list_face.assign(m->list_faces, m->list_faces + m->list_offsets[N]);
+        g.ghost_face.assign(m->ghost_face, m->ghost_face + G);
+        g.ghost_zone.assign(m->ghost_bc, m->ghost_bc + G);
+        if (m->lsq_r) g.lsq_r.assign(m->lsq_r, m->lsq_r + 6 * (size_t)N);
+        if (m->lsq_beta) g.lsq_beta.assign(m->lsq_beta, m->lsq_beta + N);
+        g.min_vol = m->min_volume;
+        g.periodic = m->face_shift != nullptr;
+        if (g.periodic) g.fshift.assign(m->face_shift, m->face_shift + 3 * (size_t)F);
+        // zones in ascending id with the canonical ranges their counts give
+        int32_t nc = 0, ng = 0, nf = 0;
+        for (int32_t i = 0; i < n_zones; ++i) {
+            const psst_zone &z = zones[i];
+            if (i > 0 && z.id <= zones[i - 1].id) throw std::invalid_argument("psst_case_from_mesh: zones not in ascending id");
+            psst::GridZone gz;
+            gz.id = z.id;
+            gz.bc = z.type;
+            gz.name = z.name ? z.name : "";
+            if (z.type == 1) {
+                gz.has_cells = true;
+                gz.cell_begin = nc;
+                gz.cell_count = z.count;
+                nc += z.count;
+            }
+            g.zones.push_back(gz);
+        }
+        for (psst::GridZone &z : g.zones) {
+            const int32_t cnt = zones[&z - g.zones.data()].count;
+            if (z.bc != 1 && z.bc != 2 && z.bc != 12 && z.bc != 31 && z.bc != 32) {
+                z.has_cells = true;
+                z.cell_begin = N + ng;
+                z.cell_count = cnt;
+                ng += cnt;
+            }
+            if (z.bc != 1 && z.bc != 31 && z.bc != 32) {
+                z.has_faces = true;
+                z.face_begin = nf;
+                z.face_count = cnt;
+                nf += cnt;
+            }
+        }
+        if (nc != N || ng != G || nf != F)
+            throw std::invalid_argument("psst_case_from_mesh: zone counts do not add up to the mesh's cells, ghosts and faces");
+        for (int32_t i = 0; i < G; ++i)
+            if (g.ghost_zone[(size_t)i] < 0 || g.ghost_zone[(size_t)i] >= n_zones)
+                throw std::invalid_argument("psst_case_from_mesh: ghost_bc out of range");
+        psst::load_settings(pc->c, case_prefix);
+        *out = pc.release();
+        return PSST_OK;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
 psst_status psst_case_mesh(const psst_case *c, psst_mesh_desc *mesh) {
     if (!c || !mesh) return PSST_BAD_ARG;
     *mesh = psst::make_mesh_desc(c->c.grid);
@@ -299,6 +383,10 @@ psst_status psst_case_write_dat(const psst_case *c, const double *prim, const ch
 
 psst_status psst_case_write_cas(const psst_case *c, const char *path) {
     if (!c || !path) return PSST_BAD_ARG;
+    if (c->raw.n_vertices() == 0) {
+        g_case_error = "psst_case_write_cas: the case has no Fluent mesh behind it (psst_case_from_mesh)";
+        return PSST_UNSUPPORTED;
+    }
     try {
         psst::write_cas(c->raw, path);
         return PSST_OK;
@@ -310,6 +398,10 @@ psst_status psst_case_write_cas(const psst_case *c, const char *path) {
 
 psst_status psst_case_solve(psst_case *c) {
     if (!c) return PSST_BAD_ARG;
+    if (c->raw.n_vertices() == 0) {
+        g_case_error = "psst_case_solve: the case has no Fluent mesh behind it (psst_case_from_mesh)";
+        return PSST_UNSUPPORTED;
+    }
     try {
         psstGridFV fv(c->c, c->raw);
         fv.solver_parameters();
