@@ -291,17 +291,31 @@ def run_fp32field(a):
     N = case.n_cells
     log("%s: %d cells" % (a.field_config, N))
 
+    def run_chunked(s, n, tol, what):
+        """s.run in chunks of at most 2000 iterations, logging each, until n
+        iterations ran or the tolerance stopped a chunk; returns the rows."""
+        rows, done = [], 0
+        while done < n:
+            k = min(2000, n - done)
+            r = s.run(k, tol=tol, allow_no_convergence=True)
+            rows.append(r[:s.iters_done])
+            done += s.iters_done
+            log("%s: %d iterations, drop %.2e" % (what, done, float((rows[-1][-1] / rows[0][0]).max())))
+            if s.iters_done < k:
+                break
+        return np.concatenate(rows), done
+
     def solve(precision, n, tol, res0=None):
         s = ss.Solver(case, precision=precision)
         if res0 is not None:
             s.set_residual_reference(res0)
         t = time.perf_counter()
-        r = s.run(n, tol=tol, allow_no_convergence=True)
+        r, _ = run_chunked(s, n, tol, "FP%d" % precision)
         el = time.perf_counter() - t
-        return s, r[:s.iters_done], el
+        return s, r, el
 
     s64, r64, t64 = solve(64, a.field_max_iter, a.tol)
-    it64 = s64.iters_done
+    it64 = len(r64)
     ref = s64.state()
     res0 = r64[0].copy()
     s64.close()
@@ -318,11 +332,9 @@ def run_fp32field(a):
         st = ss.Solver(case, precision=32)
         st.set_residual_reference(res0)
         t = time.perf_counter()
-        st.run(it64, tol=sw, allow_no_convergence=True)
-        n1 = st.iters_done
+        _, n1 = run_chunked(st, it64, sw, "two phases, FP32 to %.0e" % sw)
         st.set_precision(64)
-        st.run(a.field_max_iter, tol=a.tol, allow_no_convergence=True)
-        n2 = st.iters_done
+        _, n2 = run_chunked(st, a.field_max_iter, a.tol, "two phases, FP64")
         t_two = time.perf_counter() - t
         two.append((sw, n1, n2, t_two, st.state()))
         st.close()

@@ -12,8 +12,10 @@ the reference's host flow:
 (``psst_ctx_*``).  Errors raise ``PsstError`` (the reference throws C++
 exceptions at the same points).
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -25,6 +27,29 @@ __all__ = ["Case", "Solver", "Batch", "PsstError", "WedgeParams", "wedge_params"
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# Objects holding library handles that are still open.  At interpreter exit
+# they are closed, device objects first, while the HIP runtime is still whole:
+# Python's own finalisation may collect them in any order, or not at all,
+# and the runtime's exit handlers run after it.
+_live = weakref.WeakSet()
+
+
+def _track(obj):
+    _live.add(obj)
+    return obj
+
+
+@atexit.register
+def _close_live():
+    objs = list(_live)
+    objs.sort(key=lambda o: isinstance(o, Case))  # device contexts and batches before host cases
+    for o in objs:
+        try:
+            o.close()
+        except Exception:
+            pass
 
 
 def _check(st, ctx=None):
@@ -106,6 +131,7 @@ class Case:
 
     def __init__(self, handle):
         self._h = C.c_void_p(handle)
+        _track(self)
         info = (C.c_int64 * 8)()
         _check(lib().psst_case_info(self._h, info))
         self.dim, self.n_cells, self.n_ghosts, self.n_faces = (int(info[i]) for i in range(4))
@@ -221,6 +247,7 @@ class Solver:
         if st != 0:
             raise PsstError(st, lib().psst_last_error(None).decode())
         self._h = h
+        _track(self)
         self.set_state(case.initial_state())
 
     def close(self):
@@ -335,6 +362,7 @@ class Batch:
         if st != 0:
             raise PsstError(st, lib().psst_batch_last_error(None).decode())
         self._h = h
+        _track(self)
         self.n = n
         self.precision = int(self._settings[0].precision)
         for i, c in enumerate(self.cases):

@@ -12,7 +12,7 @@ import time
 
 import numpy as np
 
-from . import Batch, PsstError
+from . import Batch, PsstError, _track
 from ._lib import lib
 from .sweep import build_cases, draw_instances, solve_to_tolerance
 
@@ -33,6 +33,7 @@ class MLP:
         if st != 0:
             raise PsstError(st, lib().psst_mlp_last_error(None).decode())
         self._h = h
+        _track(self)
         self.dims = (n_in, width, depth, n_out)
         self.n_params = int(lib().psst_mlp_param_count(h))
 

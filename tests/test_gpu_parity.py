@@ -192,27 +192,6 @@ def test_jacobi_dt_fused_equals_unfused(tmp_path, grad):
     assert _first_mismatch(out[0], out[1]) is None
 
 
-def test_full_size_wedge_one_iteration_matches_oracle(tmp_path):
-    """The benchmark's full-size mesh (3840 x 1280 = 4,915,200 cells, jitter,
-    RCM numbering): one iteration bit-identical to the oracle, fused and
-    unfused."""
-    p = ss.wedge_params(nx=3840, ny=1280, jitter=0.1, jitter_seed=1, numbering=1)
-    prefix = str(tmp_path / "c")
-    ss.write_case_files(prefix, time_mode=2, iterations=1, flux=3, order=2, grad=1, limiter=2)
-    case = ss.Case.wedge(p, prefix)
-    print("full size: %d cells" % case.n_cells, flush=True)
-    raw = case.raw()
-    raw["dim"] = 2
-    o = Oracle(None, prefix, raw=raw)
-    ref_res = o.run(1)
-    ref = o.state()
-    for fused in (1, 0):
-        with ss.Solver(case, fused=fused) as s:
-            res = s.run(1)
-            assert np.array_equal(res, ref_res), fused
-            assert _first_mismatch(s.state(), ref) is None, (fused, _first_mismatch(s.state(), ref))
-
-
 @pytest.mark.parametrize("numbering", [1, 0])
 def test_reference_ordered_sweep_on_a_large_mesh(tmp_path, numbering):
     """Time step mode 1 in the reference's order on a mesh whose levels are
