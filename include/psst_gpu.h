@@ -145,7 +145,11 @@ psst_status psst_set_state(psst_ctx *ctx, const double *prim);
  * reference's column order (momentum..., mass, energy; :331-332, :377-382).
  * If tol > 0, stops after the first iteration whose max_j res_j/res0_j <= tol,
  * where res0 is the row set by psst_set_residual_reference (or the first row
- * this context produced); *iters_done reports how many ran.  Synchronous. */
+ * this context produced) and columns with res0_j <= 1e-12 max_k res0_k (a
+ * component zero up to rounding, such as the z momentum of a z-symmetric
+ * flow) are left out; *iters_done reports how many ran.  A non-finite
+ * residual row ends the run at that iteration (PSST_NAN_DETECTED).
+ * Synchronous. */
 psst_status psst_run(psst_ctx *ctx, int32_t n_iter, double tol, double *res_hist, int32_t *iters_done);
 
 /* Same as psst_run, allocating nothing: no hipMalloc, hipFree or other

@@ -76,13 +76,7 @@ __global__ void __launch_bounds__(64) k_run_check(RunScratch *rs, double tol) {
     if (rs->res0[0] < 0.0)
 #pragma unroll
         for (int j = 0; j < WS; ++j) rs->res0[j] = r[j];
-    double d = 0.0;
-#pragma unroll
-    for (int j = 0; j < WS; ++j) {
-        const double r0 = rs->res0[j];
-        const double q = r0 > 0.0 ? r[j] / r0 : 0.0;
-        d = d > q ? d : q;
-    }
+    const double d = residual_drop<WS>(r, rs->res0);
     if (bad) rs->ctl[2] = 1;
     if (bad || (tol > 0.0 && d <= tol)) rs->ctl[0] = it;
 }

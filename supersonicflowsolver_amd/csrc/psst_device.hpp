@@ -1621,6 +1621,28 @@ __global__ void __launch_bounds__(NTHR) k_grad_lsq_tile(Geom g, TileDev td, Stat
 // non-finite).  When the active set changed it rebuilds the list of active
 // tiles, in instance order.
 // ---------------------------------------------------------------------------
+// Residual drop max_j res_j / res0_j of a residual row against its
+// normalising row.  A column whose normalising value is round-off against
+// the row's largest (res0_j <= 1e-12 max_k res0_k, e.g. the z momentum of a
+// z-symmetric 3D flow, zero up to rounding) carries no scale and is left
+// out, as is a column with res0_j = 0.
+constexpr double kDropColumnFloor = 1.0e-12;
+template <int WS>
+__device__ __forceinline__ double residual_drop(const double (&r)[WS], const double *res0) {
+    double mx = 0.0;
+#pragma unroll
+    for (int j = 0; j < WS; ++j) mx = mx > res0[j] ? mx : res0[j];
+    const double floor = kDropColumnFloor * mx;
+    double d = 0.0;
+#pragma unroll
+    for (int j = 0; j < WS; ++j) {
+        const double r0 = res0[j];
+        const double q = (r0 > 0.0 && r0 > floor) ? r[j] / r0 : 0.0;
+        d = d > q ? d : q;
+    }
+    return d;
+}
+
 struct BatchCheck {
     int n, H;
     unsigned long long *res;  // [n][WS][kResSlots]
@@ -1702,13 +1724,7 @@ __global__ void __launch_bounds__(kCheckThreads) k_batch_check(BatchCheck b, dou
             if (b.res0[(size_t)i * WS] < 0.0)
 #pragma unroll
                 for (int j = 0; j < WS; ++j) b.res0[(size_t)i * WS + j] = r[j];
-            double d = 0.0;
-#pragma unroll
-            for (int j = 0; j < WS; ++j) {
-                const double r0 = b.res0[(size_t)i * WS + j];
-                const double q = r0 > 0.0 ? r[j] / r0 : 0.0;
-                d = d > q ? d : q;
-            }
+            const double d = residual_drop<WS>(r, b.res0 + (size_t)i * WS);
             b.drop[i] = d;
             if (bad || (tol > 0.0 && d <= tol)) {
                 b.active[i] = 0;
