@@ -670,11 +670,12 @@ def main():
     dom = max(per_iter, key=per_iter.get)
     kind = {"flux": "flux", "update": "update", "grad_gg": "grad", "grad_lsq": "grad", "bc": "bc",
             "stage": "stage", "stage_last": "stage", "update_levels_block": "update",
-            "update_levels_wg": "update", "update_levels_grid": "update", "update_level": "update"}.get(dom, dom)
+            "update_levels_wg": "update", "update_level": "update",
+            "update_chunks": "update"}.get(dom, dom)
     nbytes = alg_bytes(kind, N, G, F, nnz, dim, b, last_stage=(dom == "stage_last"))
     mean_ms = stats[dom][1]
     note = None
-    if dom in ("update_levels_block", "update_levels_wg", "update_levels_grid", "update_level"):
+    if dom in ("update_levels_block", "update_levels_wg", "update_level", "update_chunks"):
         # time step mode 1 in the reference's order: each stage's update is one sweep over the cells'
         # dependency levels (one launch, or one per level); rate over a whole stage's sweep
         mean_ms = stats[dom][0] * stats[dom][1] / (a.timing_iters * 4)

@@ -184,6 +184,20 @@ psst_status psst_set_residual_reference(psst_ctx *ctx, const double *res0);
 psst_status psst_debug_gradients(psst_ctx *ctx, double *grads);
 psst_status psst_debug_residual(psst_ctx *ctx, double *residual);
 
+/* Diagnostic, host only (no device needed): builds the schedule of the
+ * reference-ordered time step (time step mode 1, dt_ordering 0) that
+ * psst_ctx_create builds for a large mesh -- dependency levels of the
+ * in-place update loop (psstFiniteVolumeGrid.cpp:317-347 with
+ * psstLocalTimeCalculation.cpp:90-131), cells in chunks of `chunk_cells`
+ * (<= 0: the default) along the device numbering, the waits between chunks
+ * and the levels each chunk publishes -- and checks it by simulation: every
+ * earlier neighbour of a cell is finished before the cell, and every chunk
+ * finishes.  stats[9] receives: levels, chunks, entries (levels of a
+ * chunk), waits, publishes, widest entry (cells), makespan in workgroup
+ * passes with free waits and with waits costing four passes, chunk cells.
+ * PSST_BAD_ARG with psst_last_error(NULL) on a violation. */
+psst_status psst_debug_sweep_schedule(const psst_mesh_desc *mesh, int32_t chunk_cells, int64_t *stats);
+
 /* Kernel timing with HIP events on the context's stream.  While enabled,
  * every launch is bracketed by events; psst_kernel_stats returns, for the
  * kernel named `name`, the launch count and the mean duration in ms. */

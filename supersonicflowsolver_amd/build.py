@@ -11,7 +11,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpsst.so")
 
-HOST_SOURCES = ["mesh_io.cpp", "grid.cpp", "setup.cpp", "output.cpp", "meshgen.cpp", "host_api.cpp", "tiles.cpp"]
+HOST_SOURCES = ["mesh_io.cpp", "grid.cpp", "setup.cpp", "output.cpp", "meshgen.cpp", "host_api.cpp", "tiles.cpp",
+                "ordered_sched.cpp"]
 DEVICE_SOURCES = ["psst_device.hip", "mlp.hip"]
 # the solver's launch templates, instantiated once per (dimension, state type)
 INSTANCES = [(2, "double"), (2, "float"), (3, "double"), (3, "float")]

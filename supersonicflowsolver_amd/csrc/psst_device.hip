@@ -5,7 +5,10 @@
 // psst_inst.hip and only declared here.
 #include "psst_device.hpp"
 
+#include <cstdlib>
 #include <mutex>
+
+#include "ordered_sched.hpp"
 
 using namespace psst_dev;
 
@@ -17,6 +20,44 @@ PSST_DIM_T_ENTRY_POINTS(extern, 3, float)
 namespace {
 
 thread_local std::string g_create_error;
+
+// Typical cell spacing per axis: the mean |dx_k| over interior faces whose
+// centre-to-centre vector is dominated by axis k (scales the Morton curve).
+void axis_spacing(const psst_mesh_desc *m, double axis_h[3]) {
+    const int dim = m->dim, N = m->n_cells, F = m->n_faces;
+    const int64_t NT = (int64_t)m->n_cells + m->n_ghosts;
+    const int32_t *f0 = m->face_cells, *f1 = m->face_cells + F;
+    double sum[3] = {0, 0, 0};
+    int64_t cnt[3] = {0, 0, 0};
+    for (int f = 0; f < F; ++f) {
+        if (f0[f] >= N || f1[f] >= N) continue;
+        int kk = 0;
+        double best = -1.0;
+        for (int k = 0; k < dim; ++k) {
+            double d = m->cell_centre[(size_t)k * NT + f0[f]] - m->cell_centre[(size_t)k * NT + f1[f]];
+            if (m->face_shift) d -= m->face_shift[(size_t)k * F + f];
+            if (std::abs(d) > best) {
+                best = std::abs(d);
+                kk = k;
+            }
+        }
+        sum[kk] += best;
+        cnt[kk]++;
+    }
+    for (int k = 0; k < 3; ++k) axis_h[k] = (k < dim && cnt[k]) ? sum[k] / (double)cnt[k] : 0.0;
+}
+
+// Cells per chunk of the chunked ordered sweep: PSST_SWEEP_CHUNK_CELLS if
+// set (a tuning knob), else 16,384; at least N / capacity, so that the chunks
+// fit one cooperative launch (capacity 0: no limit known).
+int32_t sweep_chunk_cells(int32_t N, int capacity) {
+    int32_t k = 16384;
+    if (const char *e = std::getenv("PSST_SWEEP_CHUNK_CELLS")) {
+        const long v = std::strtol(e, nullptr, 10);
+        if (v > 0 && v <= (1L << 30)) k = (int32_t)v;
+    }
+    return capacity > 0 ? std::max<int32_t>(k, (int32_t)(((int64_t)N + capacity - 1) / capacity)) : k;
+}
 
 __global__ void k_canary(unsigned long long *p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = 0x5053535443414e59ull;  // "PSSTCANY"
@@ -217,6 +258,10 @@ psst_status run_impl(psst_ctx *c, RunScratch *rs, int32_t n_iter, double tol, do
                 for (int j = 0; j < ws; ++j) c->res0[j] = h.res0[j];
                 c->have_res0 = true;
             }
+            if (h.ctl[3] != 0) {
+                c->err = "reference-ordered sweep: a wait between chunks gave up (schedule or device fault)";
+                return PSST_HIP_ERROR;
+            }
             if (h.ctl[0] >= 0) {
                 const int last = h.ctl[0];  // stopping row of this chunk
                 c->cur = cur0 ^ (((last + 1) * c->s.rk_levels) & 1);
@@ -333,26 +378,7 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
         // typical cell spacing per axis: mean |dx_k| over interior faces whose
         // centre-to-centre vector is dominated by axis k
         double axis_h[3] = {0, 0, 0};
-        {
-            double sum[3] = {0, 0, 0};
-            int64_t cnt[3] = {0, 0, 0};
-            for (int f = 0; f < F; ++f) {
-                if (f0[f] >= N || f1[f] >= N) continue;
-                int kk = 0;
-                double best = -1.0;
-                for (int k = 0; k < dim; ++k) {
-                    double d = m->cell_centre[(size_t)k * NT + f0[f]] - m->cell_centre[(size_t)k * NT + f1[f]];
-                    if (m->face_shift) d -= m->face_shift[(size_t)k * F + f];
-                    if (std::abs(d) > best) {
-                        best = std::abs(d);
-                        kk = k;
-                    }
-                }
-                sum[kk] += best;
-                cnt[kk]++;
-            }
-            for (int k = 0; k < dim; ++k) axis_h[k] = cnt[k] ? sum[k] / (double)cnt[k] : 0.0;
-        }
+        axis_spacing(m, axis_h);
         psst::log_phase("context: mesh validated");
         std::vector<int32_t> d2c_int = psst::morton_order(dim, N, NT, m->cell_centre, seg, axis_h);
         psst::log_phase("context: cells renumbered");
@@ -481,17 +507,9 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
         // 1 + max level of its interior face-list neighbours earlier in the
         // update loop (canonical index order); stored as device ids
         if (s->time_mode == PSST_DT_LOCAL1 && s->dt_ordering == 0) {
-            std::vector<int> lev((size_t)N, 0);
+            const std::vector<int32_t> lev = psst::ordered_levels(N, m->list_offsets, lnb_c.data());
             int maxl = 0;
-            for (int i = 0; i < N; ++i) {
-                int l = 0;
-                for (int k = m->list_offsets[i]; k < m->list_offsets[i + 1]; ++k) {
-                    int nb = lnb_c[(size_t)k];
-                    if (nb < i) l = std::max(l, lev[(size_t)nb] + 1);
-                }
-                lev[(size_t)i] = l;
-                maxl = std::max(maxl, l);
-            }
+            for (int i = 0; i < N; ++i) maxl = std::max(maxl, lev[(size_t)i]);
             c->level_off.assign((size_t)maxl + 2, 0);
             for (int i = 0; i < N; ++i) c->level_off[(size_t)lev[(size_t)i] + 1]++;
             for (int l = 0; l <= maxl; ++l) c->level_off[(size_t)l + 1] += c->level_off[(size_t)l];
@@ -507,10 +525,37 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
             int W = 0;
             for (int i = 0; i < N; ++i) W = std::max(W, m->list_offsets[i + 1] - m->list_offsets[i]);
             if (seg.empty() && W <= kOrderedW) {
+                // Large meshes with wide levels sweep by chunks (one workgroup
+                // per chunk, waits only between chunks that share faces); the
+                // tables are then in chunk order, else in level order.
+                int capacity = 0;
+                if (!(widest <= 256 || N <= 65536)) {
+                    const bool f64 = s->precision == 64;
+                    capacity = dim == 2 ? (f64 ? chunk_sweep_capacity<2, double>(W) : chunk_sweep_capacity<2, float>(W))
+                                        : (f64 ? chunk_sweep_capacity<3, double>(W) : chunk_sweep_capacity<3, float>(W));
+                }
+                const bool chunked = capacity > 0;
+                psst::ChunkSchedule cs;
+                if (chunked) {
+                    cs = psst::chunk_schedule(N, sweep_chunk_cells(N, capacity), lev, m->list_offsets, lnb_c.data(),
+                                              c2d, d2c);
+                    ChunkTables &ct = c->ct;
+                    ct.ch_ent = dev_upload(c.get(), cs.ch_ent.data(), cs.ch_ent.size());
+                    ct.ent_pos = dev_upload(c.get(), cs.ent_pos.data(), cs.ent_pos.size());
+                    ct.ent_level = dev_upload(c.get(), cs.ent_level.data(), cs.ent_level.size());
+                    ct.ent_wait = dev_upload(c.get(), cs.ent_wait.data(), cs.ent_wait.size());
+                    ct.wait_chunk = dev_upload(c.get(), cs.wait_chunk.data(), cs.wait_chunk.size());
+                    ct.wait_level = dev_upload(c.get(), cs.wait_level.data(), cs.wait_level.size());
+                    ct.progress = dev_alloc<int>(c.get(), (size_t)cs.n_chunks);
+                    ct.timed_out = dev_alloc<int>(c.get(), 1);  // used outside psst_run's control words
+                    hip_check(hipMemset(ct.timed_out, 0, sizeof(int)), "hipMemset");
+                    ct.n_chunks = cs.n_chunks;
+                }
+                const std::vector<int32_t> &order = chunked ? cs.pos_cell : cells;
                 std::vector<int> onb((size_t)W * N, -1);
                 std::vector<double> ofn((size_t)dim * W * N, 0.0), ofa((size_t)W * N, 0.0), ovol((size_t)N);
                 for (int p = 0; p < N; ++p) {
-                    const int i = d2c[(size_t)cells[(size_t)p]];
+                    const int i = d2c[(size_t)order[(size_t)p]];
                     ovol[(size_t)p] = m->cell_volume[i];
                     int w = 0;
                     for (int k = m->list_offsets[i]; k < m->list_offsets[i + 1]; ++k, ++w) {
@@ -521,7 +566,7 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                             ofn[((size_t)x * W + w) * N + p] = m->face_normal[(size_t)x * F + f];
                     }
                 }
-                c->ot.cell = c->level_cells;
+                c->ot.cell = chunked ? dev_upload(c.get(), cs.pos_cell.data(), cs.pos_cell.size()) : c->level_cells;
                 c->ot.vol = dev_upload(c.get(), ovol.data(), ovol.size());
                 c->ot.nb = dev_upload(c.get(), onb.data(), onb.size());
                 c->ot.fn = dev_upload(c.get(), ofn.data(), ofn.size());
@@ -701,10 +746,11 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
         c->kernel_names = {"grad_gg",      "grad_lsq",
                            "flux",         "update",
                            "update_level", "update_levels_block",
-                           "update_levels_grid", "bc",
+                           "bc",
                            "stage",        "grad_gg_tile",
                            "grad_lsq_tile", "batch_check",
-                           "ordered_residual", "update_levels_wg"};
+                           "ordered_residual", "update_levels_wg",
+                           "update_chunks"};
     } catch (const std::exception &e) {
         g_create_error = e.what();
         for (void *p : c->allocs) (void)hipFree(p);
@@ -719,6 +765,42 @@ extern "C" {
 
 psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, psst_ctx **out) {
     return create_impl(m, s, {}, out);
+}
+
+psst_status psst_debug_sweep_schedule(const psst_mesh_desc *m, int32_t chunk_cells, int64_t *stats) {
+    if (!m || !stats || m->n_cells <= 0 || m->n_faces <= 0 || (m->dim != 2 && m->dim != 3)) {
+        g_create_error = "psst_debug_sweep_schedule: bad argument";
+        return PSST_BAD_ARG;
+    }
+    try {
+        const int N = m->n_cells, F = m->n_faces, NT = m->n_cells + m->n_ghosts;
+        const int32_t *f0 = m->face_cells, *f1 = m->face_cells + F;
+        const int nnz = m->list_offsets[N];
+        std::vector<int32_t> lnb((size_t)std::max(nnz, 1));
+        for (int i = 0; i < N; ++i)
+            for (int k = m->list_offsets[i]; k < m->list_offsets[i + 1]; ++k) {
+                const int f = m->list_faces[k];
+                if (f < 0 || f >= F || (f0[f] != i && f1[f] != i)) throw std::invalid_argument("face list corrupt");
+                lnb[(size_t)k] = f0[f] == i ? f1[f] : f0[f];
+            }
+        double axis_h[3];
+        axis_spacing(m, axis_h);
+        const std::vector<int32_t> d2c = psst::morton_order(m->dim, N, NT, m->cell_centre, {}, axis_h);
+        std::vector<int32_t> c2d((size_t)N);
+        for (int d = 0; d < N; ++d) c2d[(size_t)d2c[(size_t)d]] = d;
+        const std::vector<int32_t> lev = psst::ordered_levels(N, m->list_offsets, lnb.data());
+        const int32_t k = chunk_cells > 0 ? chunk_cells : sweep_chunk_cells(N, 0);
+        const psst::ChunkSchedule cs = psst::chunk_schedule(N, k, lev, m->list_offsets, lnb.data(), c2d, d2c);
+        const psst::ScheduleStats st =
+            psst::check_schedule(cs, N, lev, m->list_offsets, lnb.data(), c2d, d2c, kBlock);
+        const int64_t out[9] = {st.levels,       st.chunks,        st.entries,        st.waits, st.publishes,
+                                st.widest_entry, st.makespan_free, st.makespan_wait4, (int64_t)k};
+        std::memcpy(stats, out, sizeof(out));
+        return PSST_OK;
+    } catch (const std::exception &e) {
+        g_create_error = e.what();
+        return PSST_BAD_ARG;
+    }
 }
 
 psst_status psst_set_state(psst_ctx *c, const double *prim) {

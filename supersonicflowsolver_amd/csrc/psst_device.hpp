@@ -26,7 +26,6 @@
 // integer atomics on the bit patterns of non-negative doubles.
 #pragma once
 
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -95,7 +94,7 @@ constexpr int kRunRows = 1024;  // rows of one chunk
 struct RunScratch {
     unsigned long long slots[5 * 64];  // [dim+2][kResSlots]
     int ctl[4];                        // [0] row the run stopped at (-1: running), [1] rows written,
-                                       // [2] non-finite row seen, [3] unused
+                                       // [2] non-finite row seen, [3] a wait of the chunked sweep gave up
     double res0[5];                    // normalising row; res0[0] < 0: taken from the next row
     double rows[kRunRows * 5];         // [row][dim+2]
 };
@@ -804,14 +803,17 @@ constexpr int kOrderedW = 8;
 constexpr int kEarlier = 1 << 30;
 
 // Pre-pass of the ordered sweep: every cell's residual (the ordered gather
-// of update_cell) at its level-order position, and the residual max-norm;
-// neither depends on the sweep.
+// of update_cell) at its position in the sweep's tables, and the residual
+// max-norm; neither depends on the sweep.  It also clears the progress words
+// of the chunked sweep that follows it.
 template <int DIM, typename T>
 __global__ void __launch_bounds__(kBlock) k_ordered_residual(Geom g, State<T> s, OrderedTables ot,
-                                                             unsigned long long *res_row) {
+                                                             unsigned long long *res_row, int *progress,
+                                                             int n_progress) {
     if (run_stopped(g)) return;
     constexpr int WS = DIM + 2;
     const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p < n_progress) progress[p] = 0;  // chunked sweep: no chunk has published in this stage yet
     unsigned long long rmax[WS];
 #pragma unroll
     for (int j = 0; j < WS; ++j) rmax[j] = 0;
@@ -841,65 +843,107 @@ __global__ void __launch_bounds__(kBlock) k_ordered_residual(Geom g, State<T> s,
 
 // One cell of the ordered sweep from the tables: update_cell with
 // dt_local1 (psstLocalAccTime1::calculate_at, psstLocalTimeCalculation.cpp:
-// 90-131) in the same operation order.  Everything but the earlier
-// neighbours' new state is loaded straight from the position
-// (ordered_fetch), so it can be loaded before the previous level is done;
-// ordered_apply loads the neighbour states and updates the cell.
-template <int DIM, typename T>
+// 90-131) in the same operation order.  A cell is loaded in three steps, so
+// that a sweep can run them for different levels at once: ordered_head (the
+// device cell and its neighbours, by position), ordered_fetch (everything
+// else that no cell of the sweep writes: the residual, the snapshot, the
+// cell's own and its later neighbours' stage-start state, the face data) and
+// ordered_apply (the earlier neighbours' new state, then the update).  NW is
+// the table width the kernel is built for (at least the tables' W).
+template <int NW>
+struct OrderedHead {
+    int c;
+    int nbw[NW];
+};
+
+template <int DIM, typename T, int NW>
 struct OrderedIn {
     int c;
     T R[DIM + 2], w0[DIM + 2], uc[DIM], Tc, V;
-    int nbw[kOrderedW];
-    double fnw[kOrderedW][DIM], faw[kOrderedW];
+    int nbw[NW];
+    double fnw[NW][DIM], faw[NW];
+    T un[NW][DIM], Tn[NW];  // stage-start state of the neighbours that are not earlier
 };
 
-template <int DIM, typename T>
+template <int NW>
+__device__ __forceinline__ void ordered_head(const OrderedTables &ot, int N, int p, OrderedHead<NW> &h) {
+    h.c = ot.cell[p];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) h.nbw[w] = w < ot.W ? ot.nb[(size_t)w * N + p] : -1;
+}
+
+template <int DIM, typename T, int NW>
 __device__ __forceinline__ void ordered_fetch(const Geom &g, const State<T> &s, const OrderedTables &ot, int p,
-                                              OrderedIn<DIM, T> &o) {
+                                              const OrderedHead<NW> &h, OrderedIn<DIM, T, NW> &o) {
     constexpr int WS = DIM + 2;
-    o.c = ot.cell[p];
+    o.c = h.c;
     const T *Rp = static_cast<const T *>(ot.R);
 #pragma unroll
     for (int j = 0; j < WS; ++j) {
         o.R[j] = Rp[(size_t)j * g.N + p];
-        o.w0[j] = s.W0[(size_t)j * g.N + o.c];
+        o.w0[j] = s.W0[(size_t)j * g.N + h.c];
     }
 #pragma unroll
-    for (int k = 0; k < DIM; ++k) o.uc[k] = s.q[(size_t)k * g.NT + o.c];
-    o.Tc = s.q[(size_t)(DIM + 1) * g.NT + o.c];
+    for (int k = 0; k < DIM; ++k) o.uc[k] = s.q[(size_t)k * g.NT + h.c];
+    o.Tc = s.q[(size_t)(DIM + 1) * g.NT + h.c];
     o.V = T(ot.vol[p]);
 #pragma unroll
-    for (int w = 0; w < kOrderedW; ++w) {
+    for (int w = 0; w < NW; ++w) {
         const bool in = w < ot.W;
-        o.nbw[w] = in ? ot.nb[(size_t)w * g.N + p] : -1;
+        o.nbw[w] = h.nbw[w];
         o.faw[w] = in ? ot.fa[(size_t)w * g.N + p] : 0.0;
 #pragma unroll
         for (int k = 0; k < DIM; ++k) o.fnw[w][k] = in ? ot.fn[((size_t)k * ot.W + w) * g.N + p] : 0.0;
+        // a later neighbour (ghosts included) keeps its stage-start state
+        // through the sweep; padding (-1) and earlier neighbours read cell 0
+        const int nb = (h.nbw[w] < 0 || (h.nbw[w] & kEarlier)) ? 0 : h.nbw[w];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) o.un[w][k] = s.q[(size_t)k * g.NT + nb];
+        o.Tn[w] = s.q[(size_t)(DIM + 1) * g.NT + nb];
     }
 }
 
-template <int DIM, typename T>
+// Loads and stores of the new state in the chunked sweep, where workgroups
+// on different XCDs read each other's cells: agent-scope (sc1) accesses,
+// which go past the CU's L1 and keep no dirty line in the XCD's L2, so no
+// cache write-back or invalidation is needed between chunks.
+template <bool COH, typename T>
+__device__ __forceinline__ T qn_load(const T *p) {
+    if (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return *p;
+}
+template <bool COH, typename T>
+__device__ __forceinline__ void qn_store(T *p, T v) {
+    if (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+
+template <int DIM, typename T, int NW, bool COH>
 __device__ __forceinline__ void ordered_apply(const Geom &g, const State<T> &s, const Phys &ph, T alpha,
-                                              int last_stage, const OrderedIn<DIM, T> &o) {
+                                              int last_stage, const OrderedIn<DIM, T, NW> &o) {
     constexpr int WS = DIM + 2;
     const int c = o.c;
     // neighbour states: the new one for neighbours earlier in the update
-    // loop, the stage-start one otherwise (ghosts included)
-    T un[kOrderedW][DIM], Tn[kOrderedW];
+    // loop, the stage-start one (loaded ahead) otherwise
+    T un[NW][DIM], Tn[NW];
 #pragma unroll
-    for (int w = 0; w < kOrderedW; ++w) {
-        const int e = o.nbw[w] < 0 ? 0 : o.nbw[w];
-        const int nb = e & (kEarlier - 1);
-        const T *src = (e & kEarlier) ? s.qn : s.q;
+    for (int w = 0; w < NW; ++w) {
+        if (o.nbw[w] >= 0 && (o.nbw[w] & kEarlier)) {
+            const int nb = o.nbw[w] & (kEarlier - 1);
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) un[w][k] = o.nbw[w] < 0 ? T(0) : src[(size_t)k * g.NT + nb];
-        Tn[w] = o.nbw[w] < 0 ? T(0) : src[(size_t)(DIM + 1) * g.NT + nb];
+            for (int k = 0; k < DIM; ++k) un[w][k] = qn_load<COH>(s.qn + (size_t)k * g.NT + nb);
+            Tn[w] = qn_load<COH>(s.qn + (size_t)(DIM + 1) * g.NT + nb);
+        } else {
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) un[w][k] = o.un[w][k];
+            Tn[w] = o.Tn[w];
+        }
     }
     const T gr = T(ph.gamma) * T(ph.rm);
     T cc = dsqrt(gr * o.Tc);
     T den = T(0), nn = T(0);
 #pragma unroll
-    for (int w = 0; w < kOrderedW; ++w) {
+    for (int w = 0; w < NW; ++w) {
         if (o.nbw[w] < 0) break;
         T nv = T(0);
         T cn = (dsqrt(gr * Tn[w]) + cc) * T(0.5);
@@ -925,60 +969,158 @@ __device__ __forceinline__ void ordered_apply(const Geom &g, const State<T> &s, 
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
         T u = fdiv(wn[k], rho);
-        s.qn[(size_t)k * g.NT + c] = u;
+        qn_store<COH>(s.qn + (size_t)k * g.NT + c, u);
         vsq += u * u;
     }
     T Tnew = fdiv(fdiv(wn[DIM + 1], rho) - (vsq * T(0.5)), T(ph.cv));
-    s.qn[(size_t)DIM * g.NT + c] = rho * T(ph.rm) * Tnew;
-    s.qn[(size_t)(DIM + 1) * g.NT + c] = Tnew;
-    s.qn[(size_t)(DIM + 2) * g.NT + c] = rho;
+    qn_store<COH>(s.qn + (size_t)DIM * g.NT + c, rho * T(ph.rm) * Tnew);
+    qn_store<COH>(s.qn + (size_t)(DIM + 1) * g.NT + c, Tnew);
+    qn_store<COH>(s.qn + (size_t)(DIM + 2) * g.NT + c, rho);
 }
 
-template <int DIM, typename T>
+// All three steps of one position at once (cells beyond a workgroup's first
+// pass over a level, and the unpipelined sweep).
+template <int DIM, typename T, int NW, bool COH>
 __device__ __forceinline__ void ordered_update(const Geom &g, const State<T> &s, const Phys &ph, T alpha,
                                                int last_stage, const OrderedTables &ot, int p) {
-    OrderedIn<DIM, T> o;
-    ordered_fetch<DIM, T>(g, s, ot, p, o);
-    ordered_apply<DIM, T>(g, s, ph, alpha, last_stage, o);
+    OrderedHead<NW> h;
+    ordered_head<NW>(ot, g.N, p, h);
+    OrderedIn<DIM, T, NW> o;
+    ordered_fetch<DIM, T, NW>(g, s, ot, p, h, o);
+    ordered_apply<DIM, T, NW, COH>(g, s, ph, alpha, last_stage, o);
 }
 
-// The ordered sweep as one cooperative launch over several workgroups: the
-// positions of a level are spread over the grid and a grid-wide barrier
-// separates the levels.  Every workgroup runs the same number of barriers;
-// the cooperative launch guarantees that all of them are resident.
-template <int DIM, typename T>
-__global__ void __launch_bounds__(kBlock) k_update_levels_grid(Geom g, State<T> s, Phys ph, T alpha, int last_stage,
-                                                               OrderedTables ot, const int *__restrict__ level_off,
-                                                               int n_levels) {
-    if (run_stopped(g)) return;
-    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
-    const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x), nthr = (int)(gridDim.x * blockDim.x);
-    // each thread's first position of the next level is fetched right after
-    // its cell of this level is updated (nothing it reads is written by this
-    // level), so its loads overlap the stores the grid barrier waits for
-    OrderedIn<DIM, T> o;
-    bool have = n_levels > 0 && level_off[0] + tid < level_off[1];
-    if (have) ordered_fetch<DIM, T>(g, s, ot, level_off[0] + tid, o);
-    for (int l = 0; l < n_levels; ++l) {
-        const int b = level_off[l], e = level_off[l + 1];
-        if (have) ordered_apply<DIM, T>(g, s, ph, alpha, last_stage, o);
-        for (int p = b + tid + nthr; p < e; p += nthr) ordered_update<DIM, T>(g, s, ph, alpha, last_stage, ot, p);
-        have = false;
-        if (l + 1 < n_levels) {
-            const int pn = level_off[l + 1] + tid;
-            if (pn < level_off[l + 2]) {
-                ordered_fetch<DIM, T>(g, s, ot, pn, o);
-                have = true;
+// The three-step pipeline of a workgroup walking a sequence of levels, with
+// one thread per position on the first pass over each level: while the
+// cells of level e are updated, the thread's position of level e + 1 is
+// fetched (its head was loaded one level earlier) and the head of its
+// position of level e + 2 is loaded.  lev_pos(e) is the first position of
+// level e of the sequence (e = n_lev: one past the last).
+template <int DIM, typename T, int NW>
+struct SweepPipe {
+    OrderedIn<DIM, T, NW> o;  // level e
+    OrderedHead<NW> h;        // level e + 1
+    bool have = false, have_h = false;
+    int ph = 0;               // position of the head
+
+    template <typename LevPos>
+    __device__ __forceinline__ void start(const Geom &g, const State<T> &s, const OrderedTables &ot, int n_lev,
+                                          int tid, LevPos lev_pos) {
+        if (n_lev > 0) {
+            const int p = lev_pos(0) + tid;
+            have = p < lev_pos(1);
+            if (have) {
+                OrderedHead<NW> h0;
+                ordered_head<NW>(ot, g.N, p, h0);
+                ordered_fetch<DIM, T, NW>(g, s, ot, p, h0, o);
             }
         }
-        grid.sync();
+        load_head(ot, g.N, 1, n_lev, tid, lev_pos);
+    }
+    template <typename LevPos>
+    __device__ __forceinline__ void load_head(const OrderedTables &ot, int N, int e, int n_lev, int tid,
+                                              LevPos lev_pos) {
+        have_h = false;
+        if (e < n_lev) {
+            ph = lev_pos(e) + tid;
+            if (ph < lev_pos(e + 1)) {
+                ordered_head<NW>(ot, N, ph, h);
+                have_h = true;
+            }
+        }
+    }
+    // after level e's updates: fetch level e + 1, load the head of e + 2
+    template <typename LevPos>
+    __device__ __forceinline__ void advance(const Geom &g, const State<T> &s, const OrderedTables &ot, int e,
+                                            int n_lev, int tid, LevPos lev_pos) {
+        have = have_h;
+        if (have) ordered_fetch<DIM, T, NW>(g, s, ot, ph, h, o);
+        load_head(ot, g.N, e + 2, n_lev, tid, lev_pos);
+    }
+};
+
+// Tables of the chunked ordered sweep (host side: ordered_sched.cpp).  One
+// workgroup per chunk of cells walks the chunk's entries (one entry = the
+// chunk's cells of one level, at positions [ent_pos[e], ent_pos[e+1]) of the
+// OrderedTables).  Before an entry it waits until every chunk
+// wait_chunk[i], i in [ent_wait[e], ent_wait[e+1]), has finished level
+// wait_level[i]; after an entry whose ent_level carries kChunkPublish it
+// publishes progress = level + 1.
+struct ChunkTables {
+    const int *__restrict__ ch_ent;      // [n_chunks+1]
+    const int *__restrict__ ent_pos;     // [E+1]
+    const int *__restrict__ ent_level;   // [E] level | kChunkPublish
+    const int *__restrict__ ent_wait;    // [E+1]
+    const int *__restrict__ wait_chunk;  // [waits]
+    const int *__restrict__ wait_level;  // [waits]
+    int *progress;                       // [n_chunks]: 1 + the last level published, 0 at the start of a stage
+    int *timed_out;                      // set to 1 when a wait gives up; then every wait gives up
+    int n_chunks;
+};
+constexpr int kChunkPublish = 1 << 30;
+// Polls of one wait before it gives up (a second or two): a wait ends once
+// the chunk it waits for publishes, which the schedule guarantees; the bound
+// only keeps a broken schedule from holding the device.  The run then fails.
+constexpr int kChunkSpinLimit = 1 << 21;
+
+// The ordered sweep as one cooperative launch of one workgroup per chunk
+// (all resident, so every wait ends).  A chunk's levels are separated by a
+// workgroup barrier (the CU's own accesses reach the L2 in order).  Across
+// chunks the new state goes through agent-scope loads and stores: before a
+// publishing barrier every wave waits until its stores are complete
+// (s_waitcnt vmcnt(0)), so the progress word thread 0 then stores is seen
+// only after the cells it covers.  The entries are pipelined (SweepPipe),
+// so a level's chain is only the earlier neighbours' new state and the
+// update.
+template <int DIM, typename T, int NW>
+__global__ void __launch_bounds__(kBlock) k_update_chunks(Geom g, State<T> s, Phys ph, T alpha, int last_stage,
+                                                          OrderedTables ot, ChunkTables ct) {
+    if (run_stopped(g)) return;
+    const int w = (int)blockIdx.x, tid = (int)threadIdx.x, nthr = (int)blockDim.x;
+    const int e0 = ct.ch_ent[w], e1 = ct.ch_ent[w + 1];
+    const int *ent_pos = ct.ent_pos + e0;
+    auto lev_pos = [ent_pos](int e) { return ent_pos[e]; };
+    SweepPipe<DIM, T, NW> pipe;
+    pipe.start(g, s, ot, e1 - e0, tid, lev_pos);
+    for (int e = e0; e < e1; ++e) {
+        const int wb = ct.ent_wait[e], we = ct.ent_wait[e + 1];
+        if (wb < we) {
+            for (int i = wb + tid; i < we; i += nthr) {
+                const int need = ct.wait_level[i] + 1;
+                const int *flag = ct.progress + ct.wait_chunk[i];
+                for (int spins = 0; __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need;
+                     ++spins) {
+                    // once any wait has given up, every other one gives up at once
+                    if ((spins & 1023) == 1023 &&
+                        __hip_atomic_load(ct.timed_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                        break;
+                    if (spins > kChunkSpinLimit) {
+                        __hip_atomic_store(ct.timed_out, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __syncthreads();
+        }
+        const int b = ct.ent_pos[e], en = ct.ent_pos[e + 1];
+        if (pipe.have) ordered_apply<DIM, T, NW, true>(g, s, ph, alpha, last_stage, pipe.o);
+        for (int p = b + tid + nthr; p < en; p += nthr)
+            ordered_update<DIM, T, NW, true>(g, s, ph, alpha, last_stage, ot, p);
+        const int lv = ct.ent_level[e];
+        if (lv & kChunkPublish) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        pipe.advance(g, s, ot, e - e0, e1 - e0, tid, lev_pos);
+        __syncthreads();
+        if ((lv & kChunkPublish) && tid == 0)
+            __hip_atomic_store(ct.progress + w, (lv & (kChunkPublish - 1)) + 1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
 // The ordered sweep in one workgroup, a workgroup barrier between levels
 // (small meshes or narrow levels).  With the residual gathered by the
 // parallel pre-pass, each level is only the time step and update chain.
-template <int DIM, typename T>
+template <int DIM, typename T, int NW>
 __global__ void __launch_bounds__(1024) k_update_levels_wg(Geom g, State<T> s, Phys ph, T alpha, int last_stage,
                                                            OrderedTables ot, const int *__restrict__ level_off,
                                                            int n_levels) {
@@ -986,37 +1128,30 @@ __global__ void __launch_bounds__(1024) k_update_levels_wg(Geom g, State<T> s, P
     for (int l = 0; l < n_levels; ++l) {
         const int b = level_off[l], e = level_off[l + 1];
         for (int p = b + (int)threadIdx.x; p < e; p += (int)blockDim.x)
-            ordered_update<DIM, T>(g, s, ph, alpha, last_stage, ot, p);
+            ordered_update<DIM, T, NW, false>(g, s, ph, alpha, last_stage, ot, p);
         __syncthreads();
     }
 }
 
-// Same for levels of at most 256 cells, with each thread's position of the
-// next level fetched as soon as its cell of this level is updated: a
-// workgroup barrier waits for LDS operations only, so those loads stay in
-// flight across it.  Nothing they read is written by the level before
+// Same for levels of at most 256 cells, pipelined (SweepPipe): a workgroup
+// barrier waits for LDS operations only, so the loads of the next two levels
+// stay in flight across it.  Nothing they read is written during the sweep
 // (tables, residuals, the stage-start state, the cell's own snapshot).
-template <int DIM, typename T>
+template <int DIM, typename T, int NW>
 __global__ void __launch_bounds__(256) k_update_levels_wg_prefetch(Geom g, State<T> s, Phys ph, T alpha,
                                                                    int last_stage, OrderedTables ot,
                                                                    const int *__restrict__ level_off, int n_levels) {
     if (run_stopped(g)) return;
     const int tid = (int)threadIdx.x, nthr = (int)blockDim.x;
-    OrderedIn<DIM, T> o;
-    bool have = n_levels > 0 && level_off[0] + tid < level_off[1];
-    if (have) ordered_fetch<DIM, T>(g, s, ot, level_off[0] + tid, o);
+    auto lev_pos = [level_off](int e) { return level_off[e]; };
+    SweepPipe<DIM, T, NW> pipe;
+    pipe.start(g, s, ot, n_levels, tid, lev_pos);
     for (int l = 0; l < n_levels; ++l) {
         const int b = level_off[l], e = level_off[l + 1];
-        if (have) ordered_apply<DIM, T>(g, s, ph, alpha, last_stage, o);
-        for (int p = b + tid + nthr; p < e; p += nthr) ordered_update<DIM, T>(g, s, ph, alpha, last_stage, ot, p);
-        have = false;
-        if (l + 1 < n_levels) {
-            const int pn = level_off[l + 1] + tid;
-            if (pn < level_off[l + 2]) {
-                ordered_fetch<DIM, T>(g, s, ot, pn, o);
-                have = true;
-            }
-        }
+        if (pipe.have) ordered_apply<DIM, T, NW, false>(g, s, ph, alpha, last_stage, pipe.o);
+        for (int p = b + tid + nthr; p < e; p += nthr)
+            ordered_update<DIM, T, NW, false>(g, s, ph, alpha, last_stage, ot, p);
+        pipe.advance(g, s, ot, l, n_levels, tid, lev_pos);
         __syncthreads();
     }
 }
@@ -1798,8 +1933,9 @@ struct psst_ctx {
     int *level_cells = nullptr, *level_off_dev = nullptr;
     std::vector<int> level_off;
     int max_level = 0;   // cells in the widest level
-    int coop_blocks = -1;  // workgroups of the cooperative sweep (-1: not sized yet, 0: unavailable)
-    psst_dev::OrderedTables ot{};    // tables of the cooperative sweep (ot.cell == nullptr: not built)
+    psst_dev::OrderedTables ot{};    // tables of the ordered sweeps (ot.cell == nullptr: not built)
+    psst_dev::ChunkTables ct{};      // chunked sweep (ct.n_chunks == 0: not built; ot then in level order)
+    int chunk_ok[2] = {-1, -1};      // the chunks fit one cooperative launch, FP32 / FP64 (-1: not checked)
     // residual normalisation for the tolerance test
     double res0[5] = {0, 0, 0, 0, 0};
     bool have_res0 = false;
@@ -2124,21 +2260,63 @@ void run_flux(psst_ctx *c) {
     }
 }
 
-// Workgroups of the cooperative ordered sweep: enough to hold the widest
-// level at one cell per thread, within what the device keeps resident at once;
-// 0 where cooperative launches are unavailable.
+// Workgroups that one cooperative launch of the chunked ordered sweep holds
+// on the current device for tables of width W (0: no cooperative launches).
 template <int DIM, typename T>
-int coop_sweep_blocks(psst_ctx *c) {
-    if (c->coop_blocks >= 0) return c->coop_blocks;
+int chunk_sweep_capacity(int W) {
     int dev = 0, coop = 0, cus = 0, per_cu = 0;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     hip_check(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev), "hipDeviceGetAttribute");
     hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
-    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_update_levels_grid<DIM, T>, kBlock, 0),
-              "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-    const int want = (c->max_level + kBlock - 1) / kBlock;
-    c->coop_blocks = coop ? std::max(0, std::min(want, cus * per_cu)) : 0;
-    return c->coop_blocks;
+    if (W <= 4)
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_update_chunks<DIM, T, 4>, kBlock, 0),
+                  "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    else
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_update_chunks<DIM, T, kOrderedW>, kBlock, 0),
+                  "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    return coop ? cus * per_cu : 0;
+}
+
+// Whether one cooperative launch holds a workgroup per chunk of the chunked
+// ordered sweep (every chunk must be resident for its waits to end).
+template <int DIM, typename T>
+bool chunk_sweep_fits(psst_ctx *c) {
+    int &ok = c->chunk_ok[sizeof(T) == 8 ? 1 : 0];
+    if (ok < 0) ok = (c->ct.n_chunks > 0 && c->ct.n_chunks <= chunk_sweep_capacity<DIM, T>(c->ot.W)) ? 1 : 0;
+    return ok == 1;
+}
+
+// The reference-ordered update of one stage from the tables (time step mode
+// 1, dt_ordering 0): the residuals in parallel, then the dependent chain of
+// time steps and updates.  Large meshes: one workgroup per chunk walking its
+// levels, each waiting only for the chunks that own earlier neighbours of
+// its next level.  Small meshes: one workgroup walking every level, with a
+// workgroup barrier per level (the reference's own 19,200-cell case).
+// Returns false where the tables cannot be used.
+template <int DIM, typename T, int NW>
+bool run_ordered_tables(psst_ctx *c, const State<T> &st, T alpha, int last, unsigned long long *res_row,
+                        bool narrow) {
+    const int nl = (int)c->level_off.size() - 1;
+    if (c->ct.n_chunks > 0) {
+        if (!chunk_sweep_fits<DIM, T>(c)) return false;
+        ChunkTables ct = c->ct;  // a wait that gives up is reported in the run's control word 3
+        if (c->geom.stop) ct.timed_out = const_cast<int *>(c->geom.stop) + 3;
+        launch(c, "ordered_residual", k_ordered_residual<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st,
+               c->ot, res_row, ct.progress, ct.n_chunks);
+        launch_coop(c, "update_chunks", k_update_chunks<DIM, T, NW>, dim3(ct.n_chunks), dim3(kBlock), c->geom, st,
+                    c->ph, alpha, last, c->ot, ct);
+        return true;
+    }
+    if (!narrow) return false;
+    launch(c, "ordered_residual", k_ordered_residual<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st, c->ot,
+           res_row, (int *)nullptr, 0);
+    if (c->max_level <= 256)
+        launch(c, "update_levels_wg", k_update_levels_wg_prefetch<DIM, T, NW>, dim3(1), dim3(256), c->geom, st, c->ph,
+               alpha, last, c->ot, (const int *)c->level_off_dev, nl);
+    else
+        launch(c, "update_levels_wg", k_update_levels_wg<DIM, T, NW>, dim3(1), dim3(1024), c->geom, st, c->ph, alpha,
+               last, c->ot, (const int *)c->level_off_dev, nl);
+    return true;
 }
 
 template <int DIM, typename T>
@@ -2148,29 +2326,16 @@ void run_update(psst_ctx *c, int k, unsigned long long *res_row) {
     const int first = (k == 0), last = (k == c->s.rk_levels - 1);
     if (c->s.time_mode == PSST_DT_LOCAL1 && c->s.dt_ordering == 0) {
         const int nl = (int)c->level_off.size() - 1;
-        // One workgroup sweeping every level (a barrier per level) suits small
-        // meshes; on large ones it runs on one CU: 938 ms per iteration on the
-        // 4.9M-cell wedge (about 5,000 levels of up to 1,280 cells), against
-        // 325 ms with one multi-workgroup launch per level.
+        // One workgroup sweeping every level suits small meshes; on large
+        // ones it runs on one CU: 938 ms per iteration on the 4.9M-cell wedge
+        // (about 5,000 levels of up to 1,280 cells), against 325 ms with one
+        // multi-workgroup launch per level, the fallback here.
         const bool narrow = c->max_level <= 256 || c->N <= 65536;
-        if (!c->batch && c->ot.cell && (narrow || coop_sweep_blocks<DIM, T>(c) > 0)) {
-            // residuals in parallel, then the dependent chain of time steps
-            // and updates level by level: in one workgroup when the levels are
-            // narrow (the reference's own 19,200-cell case: 1.8 ms per
-            // iteration before, with the residual gather inside the chain),
-            // else in one cooperative launch with a grid barrier per level
-            launch(c, "ordered_residual", k_ordered_residual<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st,
-                   c->ot, res_row);
-            if (narrow && c->max_level <= 256)
-                launch(c, "update_levels_wg", k_update_levels_wg_prefetch<DIM, T>, dim3(1), dim3(256), c->geom, st,
-                       c->ph, alpha, last, c->ot, (const int *)c->level_off_dev, nl);
-            else if (narrow)
-                launch(c, "update_levels_wg", k_update_levels_wg<DIM, T>, dim3(1), dim3(1024), c->geom, st, c->ph,
-                       alpha, last, c->ot, (const int *)c->level_off_dev, nl);
-            else
-                launch_coop(c, "update_levels_grid", k_update_levels_grid<DIM, T>, dim3(c->coop_blocks),
-                            dim3(kBlock), c->geom, st, c->ph, alpha, last, c->ot, (const int *)c->level_off_dev, nl);
-        } else if (narrow) {
+        if (!c->batch && c->ot.cell &&
+            (c->ot.W <= 4 ? run_ordered_tables<DIM, T, 4>(c, st, alpha, last, res_row, narrow)
+                          : run_ordered_tables<DIM, T, kOrderedW>(c, st, alpha, last, res_row, narrow)))
+            return;
+        if (narrow) {
             launch(c, "update_levels_block", k_update_levels_block<DIM, T>, dim3(1), dim3(1024), c->geom, st, c->ph,
                    alpha, first, last, res_row, (const int *)c->level_cells, (const int *)c->level_off_dev, nl);
         } else {
@@ -2240,4 +2405,5 @@ void run_debug_residual(psst_ctx *c, double *out) {
     EXT template void psst_dev::run_iteration<DIM, T>(psst_ctx *, unsigned long long *);     \
     EXT template void psst_dev::run_set_w<DIM, T>(psst_ctx *, int, int);                      \
     EXT template void psst_dev::run_grad<DIM, T>(psst_ctx *);                                 \
-    EXT template void psst_dev::run_debug_residual<DIM, T>(psst_ctx *, double *);
+    EXT template void psst_dev::run_debug_residual<DIM, T>(psst_ctx *, double *);         \
+    EXT template int psst_dev::chunk_sweep_capacity<DIM, T>(int);

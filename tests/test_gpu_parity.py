@@ -196,8 +196,9 @@ def test_jacobi_dt_fused_equals_unfused(tmp_path, grad):
 def test_reference_ordered_sweep_on_a_large_mesh(tmp_path, numbering):
     """Time step mode 1 in the reference's order on a mesh whose levels are
     wider than one workgroup's sweep (800 x 400 = 320,000 cells): the
-    cooperative sweep (one launch per stage, a grid barrier per level) is
-    bit-identical to the oracle over three iterations."""
+    chunked sweep (one cooperative launch per stage, a workgroup per chunk of
+    cells, waits only between chunks) is bit-identical to the oracle over
+    three iterations."""
     p = ss.wedge_params(nx=800, ny=400, jitter=0.1, jitter_seed=4, numbering=numbering)
     prefix = str(tmp_path / "c")
     ss.write_case_files(prefix, time_mode=1, iterations=3, flux=3, order=2, grad=1, limiter=2, cfl=0.8)
@@ -210,15 +211,15 @@ def test_reference_ordered_sweep_on_a_large_mesh(tmp_path, numbering):
     with ss.Solver(case) as s:
         s.kernel_timing(True)
         res = s.run(3)
-        launches = s.kernel_stats("update_levels_grid")[0]
-        print("numbering %d: %d cooperative sweeps" % (numbering, launches), flush=True)
+        launches = s.kernel_stats("update_chunks")[0]
+        print("numbering %d: %d chunked sweeps" % (numbering, launches), flush=True)
         assert launches == 3 * 4
         assert np.array_equal(res, ref_res)
         assert _first_mismatch(s.state(), ref) is None, _first_mismatch(s.state(), ref)
 
 
 def test_reference_ordered_sweep_3d_and_fp32(tmp_path):
-    """The cooperative ordered sweep on a 3D hexahedral mesh (56 x 32 x 48 =
+    """The chunked ordered sweep on a 3D hexahedral mesh (56 x 32 x 48 =
     86,016 cells, levels of several hundred cells): bit-identical to the
     oracle in FP64; in FP32 within 1e-5 (relative L2) of it after three
     iterations."""
@@ -236,8 +237,8 @@ def test_reference_ordered_sweep_3d_and_fp32(tmp_path):
         with ss.Solver(case, precision=precision) as s:
             s.kernel_timing(True)
             res = s.run(3)
-            launches = s.kernel_stats("update_levels_grid")[0]
-            print("precision %d: %d cooperative sweeps" % (precision, launches), flush=True)
+            launches = s.kernel_stats("update_chunks")[0]
+            print("precision %d: %d chunked sweeps" % (precision, launches), flush=True)
             assert launches == 3 * 4
             st = s.state()
         if precision == 64:
