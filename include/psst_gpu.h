@@ -188,15 +188,19 @@ psst_status psst_debug_residual(psst_ctx *ctx, double *residual);
  * reference-ordered time step (time step mode 1, dt_ordering 0) that
  * psst_ctx_create builds for a large mesh -- dependency levels of the
  * in-place update loop (psstFiniteVolumeGrid.cpp:317-347 with
- * psstLocalTimeCalculation.cpp:90-131), cells in chunks of `chunk_cells`
- * (<= 0: the default) along the device numbering, the waits between chunks
- * and the levels each chunk publishes -- and checks it by simulation: every
- * earlier neighbour of a cell is finished before the cell, and every chunk
- * finishes.  stats[9] receives: levels, chunks, entries (levels of a
- * chunk), waits, publishes, widest entry (cells), makespan in workgroup
- * passes with free waits and with waits costing four passes, chunk cells.
- * PSST_BAD_ARG with psst_last_error(NULL) on a violation. */
-psst_status psst_debug_sweep_schedule(const psst_mesh_desc *mesh, int32_t chunk_cells, int64_t *stats);
+ * psstLocalTimeCalculation.cpp:90-131), cells in chunks of about
+ * `chunk_cells` (<= 0: the default), the waits between chunks and the
+ * levels each chunk publishes -- and checks it by simulation: every earlier
+ * neighbour of a cell is finished before the cell, and every chunk
+ * finishes.  partition: 1 = ranges of the device (Morton) numbering, 2 =
+ * boxes of per-axis dependency potentials, 0 = the one with the shorter
+ * modelled critical path (as psst_ctx_create picks).  stats[10] receives:
+ * levels, chunks, entries (levels of a chunk), waits, publishes, widest
+ * entry (cells), makespan in workgroup passes with free waits and with
+ * waits costing four passes, chunk cells, partition used.  PSST_BAD_ARG
+ * with psst_last_error(NULL) on a violation. */
+psst_status psst_debug_sweep_schedule(const psst_mesh_desc *mesh, int32_t chunk_cells, int32_t partition,
+                                      int64_t *stats);
 
 /* Kernel timing with HIP events on the context's stream.  While enabled,
  * every launch is bracketed by events; psst_kernel_stats returns, for the

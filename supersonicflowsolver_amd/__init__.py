@@ -475,15 +475,16 @@ def autotune(case, settings=None, prim=None, trial_iters=10, **overrides):
     return (best[0], best[1]), table
 
 
-def sweep_schedule(case, chunk_cells=0):
+def sweep_schedule(case, chunk_cells=0, partition=0):
     """Builds and checks (on the host) the chunk schedule of the reference-ordered
     time step that a device context uses on a large mesh
-    (``psst_debug_sweep_schedule``); returns its figures as a dict."""
+    (``psst_debug_sweep_schedule``; partition 1 = Morton ranges, 2 = potential
+    boxes, 0 = the context's choice); returns its figures as a dict."""
     m = case.mesh()
-    st = (C.c_int64 * 9)()
-    code = lib().psst_debug_sweep_schedule(C.byref(m), int(chunk_cells), st)
+    st = (C.c_int64 * 10)()
+    code = lib().psst_debug_sweep_schedule(C.byref(m), int(chunk_cells), int(partition), st)
     if code != 0:
         raise PsstError(code, lib().psst_last_error(None).decode())
     keys = ("levels", "chunks", "entries", "waits", "publishes", "widest_entry", "makespan_free",
-            "makespan_wait4", "chunk_cells")
+            "makespan_wait4", "chunk_cells", "partition")
     return dict(zip(keys, (int(x) for x in st)))

@@ -72,7 +72,7 @@ _SIGS = {
     "psst_set_residual_reference": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "psst_debug_gradients": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "psst_debug_residual": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
-    "psst_debug_sweep_schedule": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, C.POINTER(C.c_int64)]),
+    "psst_debug_sweep_schedule": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, C.c_int32, C.POINTER(C.c_int64)]),
     "psst_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "psst_kernel_stats": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "psst_kernel_names": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),

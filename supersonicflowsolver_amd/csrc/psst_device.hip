@@ -48,15 +48,62 @@ void axis_spacing(const psst_mesh_desc *m, double axis_h[3]) {
 }
 
 // Cells per chunk of the chunked ordered sweep: PSST_SWEEP_CHUNK_CELLS if
-// set (a tuning knob), else 16,384; at least N / capacity, so that the chunks
-// fit one cooperative launch (capacity 0: no limit known).
+// set (a tuning knob), else 8,192; at least N / capacity, so that the chunks
+// fit one cooperative launch (capacity 0: no limit known).  On the
+// 4.9M-cell wedge 9,600 cells per chunk (the capacity bound) sweep a stage
+// in 15.7 ms, 16,384 in 16.5 ms and 32,768 in 19.1 ms.
 int32_t sweep_chunk_cells(int32_t N, int capacity) {
-    int32_t k = 16384;
+    int32_t k = 8192;
     if (const char *e = std::getenv("PSST_SWEEP_CHUNK_CELLS")) {
         const long v = std::strtol(e, nullptr, 10);
         if (v > 0 && v <= (1L << 30)) k = (int32_t)v;
     }
     return capacity > 0 ? std::max<int32_t>(k, (int32_t)(((int64_t)N + capacity - 1) / capacity)) : k;
+}
+
+// Levels between publications of a chunk's progress:
+// PSST_SWEEP_PUBLISH_STRIDE if set (a tuning knob), else 4.
+int32_t sweep_publish_stride() {
+    if (const char *e = std::getenv("PSST_SWEEP_PUBLISH_STRIDE")) {
+        const long v = std::strtol(e, nullptr, 10);
+        if (v > 0 && v <= 1024) return (int32_t)v;
+    }
+    return 4;
+}
+
+// Chunk schedule of the reference-ordered sweep with chunks of about
+// chunk_cells cells: partition 1 = ranges of the device numbering (Morton),
+// 2 = boxes of the per-axis potentials (no two chunks wait for each other),
+// 0 = whichever gives the shorter modelled critical path with at most
+// max_chunks chunks (0: no limit).  *used receives the partition taken.
+psst::ChunkSchedule plan_sweep(const psst_mesh_desc *m, int32_t chunk_cells, int partition, int max_chunks,
+                               const std::vector<int32_t> &lev, const int32_t *lnb, const std::vector<int32_t> &c2d,
+                               const std::vector<int32_t> &d2c, int *used) {
+    const int N = m->n_cells;
+    psst::ChunkSchedule best;
+    int64_t best_span = -1;
+    for (int part : {1, 2}) {
+        if (partition != 0 && partition != part) continue;
+        std::vector<int32_t> chunk_of;
+        const int32_t n = part == 1 ? psst::morton_chunks(N, chunk_cells, c2d, chunk_of)
+                                    : psst::potential_chunks(N, m->dim, chunk_cells, m->list_offsets, m->list_faces,
+                                                             lnb, m->face_normal, m->n_faces, chunk_of);
+        if (partition == 0 && max_chunks > 0 && n > max_chunks) continue;
+        // a publication stride above 1 can close a cycle of waits between chunks that
+        // wait for each other; such a schedule publishes after every waited-for level
+        const int32_t stride = sweep_publish_stride();
+        psst::ChunkSchedule cs = psst::chunk_schedule(N, n, chunk_of, lev, m->list_offsets, lnb, c2d, d2c, stride);
+        if (stride > 1 && !psst::schedule_completes(cs))
+            cs = psst::chunk_schedule(N, n, chunk_of, lev, m->list_offsets, lnb, c2d, d2c, 1);
+        const int64_t span = partition == 0 ? psst::schedule_stats(cs, kBlock).makespan_wait4 : 0;
+        if (best_span < 0 || span < best_span) {
+            best = std::move(cs);
+            best_span = span;
+            *used = part;
+        }
+    }
+    if (best_span < 0) throw std::invalid_argument("no chunk partition within the workgroup limit");
+    return best;
 }
 
 __global__ void k_canary(unsigned long long *p) {
@@ -525,6 +572,7 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
             int W = 0;
             for (int i = 0; i < N; ++i) W = std::max(W, m->list_offsets[i + 1] - m->list_offsets[i]);
             if (seg.empty() && W <= kOrderedW) {
+                W = W <= 4 ? 4 : kOrderedW;  // the width the sweep kernels are built for
                 // Large meshes with wide levels sweep by chunks (one workgroup
                 // per chunk, waits only between chunks that share faces); the
                 // tables are then in chunk order, else in level order.
@@ -534,18 +582,37 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                     capacity = dim == 2 ? (f64 ? chunk_sweep_capacity<2, double>(W) : chunk_sweep_capacity<2, float>(W))
                                         : (f64 ? chunk_sweep_capacity<3, double>(W) : chunk_sweep_capacity<3, float>(W));
                 }
-                const bool chunked = capacity > 0;
+                bool chunked = capacity > 0 && NT < kRemote;
                 psst::ChunkSchedule cs;
+                std::vector<int32_t> chunk_of_d;  // chunk of every device interior cell
                 if (chunked) {
-                    cs = psst::chunk_schedule(N, sweep_chunk_cells(N, capacity), lev, m->list_offsets, lnb_c.data(),
-                                              c2d, d2c);
+                    int used = 0;
+                    cs = plan_sweep(m, sweep_chunk_cells(N, capacity), 0, capacity, lev, lnb_c.data(), c2d, d2c,
+                                    &used);
+                    // a chunk's entry tables must fit the kernel's LDS
+                    for (int w = 0; w < cs.n_chunks && chunked; ++w) {
+                        const int32_t e0 = cs.ch_ent[(size_t)w], e1 = cs.ch_ent[(size_t)w + 1];
+                        chunked = e1 - e0 <= kChunkMaxEntries &&
+                                  cs.ent_wait[(size_t)e1] - cs.ent_wait[(size_t)e0] <= kChunkMaxWaits;
+                    }
+                    psst::log_phase(!chunked ? "context: chunk tables too large, ordered sweep by levels"
+                                    : used == 1 ? "context: ordered sweep by Morton chunks"
+                                                : "context: ordered sweep by potential boxes");
+                }
+                if (chunked) {
+                    chunk_of_d.resize((size_t)N);
+                    for (int w = 0; w < cs.n_chunks; ++w)
+                        for (int p = cs.ent_pos[(size_t)cs.ch_ent[(size_t)w]];
+                             p < cs.ent_pos[(size_t)cs.ch_ent[(size_t)w + 1]]; ++p)
+                            chunk_of_d[(size_t)cs.pos_cell[(size_t)p]] = w;
+                    std::vector<int2> waits(cs.wait_chunk.size());
+                    for (size_t i = 0; i < waits.size(); ++i) waits[i] = make_int2(cs.wait_chunk[i], cs.wait_level[i]);
                     ChunkTables &ct = c->ct;
                     ct.ch_ent = dev_upload(c.get(), cs.ch_ent.data(), cs.ch_ent.size());
                     ct.ent_pos = dev_upload(c.get(), cs.ent_pos.data(), cs.ent_pos.size());
                     ct.ent_level = dev_upload(c.get(), cs.ent_level.data(), cs.ent_level.size());
                     ct.ent_wait = dev_upload(c.get(), cs.ent_wait.data(), cs.ent_wait.size());
-                    ct.wait_chunk = dev_upload(c.get(), cs.wait_chunk.data(), cs.wait_chunk.size());
-                    ct.wait_level = dev_upload(c.get(), cs.wait_level.data(), cs.wait_level.size());
+                    ct.waits = dev_upload(c.get(), waits.data(), waits.size());
                     ct.progress = dev_alloc<int>(c.get(), (size_t)cs.n_chunks);
                     ct.timed_out = dev_alloc<int>(c.get(), 1);  // used outside psst_run's control words
                     hip_check(hipMemset(ct.timed_out, 0, sizeof(int)), "hipMemset");
@@ -560,13 +627,34 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                     int w = 0;
                     for (int k = m->list_offsets[i]; k < m->list_offsets[i + 1]; ++k, ++w) {
                         const int f = m->list_faces[k], nbc = lnb_c[(size_t)k];
-                        onb[(size_t)w * N + p] = c2d[(size_t)nbc] | (nbc < i ? kEarlier : 0);
+                        int e = c2d[(size_t)nbc];
+                        if (nbc < i) {
+                            e |= kEarlier;
+                            if (chunked && chunk_of_d[(size_t)c2d[(size_t)nbc]] != chunk_of_d[(size_t)order[(size_t)p]])
+                                e |= kRemote;
+                        }
+                        onb[(size_t)w * N + p] = e;
                         ofa[(size_t)w * N + p] = m->face_area[f];
                         for (int x = 0; x < dim; ++x)
                             ofn[((size_t)x * W + w) * N + p] = m->face_normal[(size_t)x * F + f];
                     }
                 }
-                c->ot.cell = chunked ? dev_upload(c.get(), cs.pos_cell.data(), cs.pos_cell.size()) : c->level_cells;
+                if (chunked) {
+                    // a cell is a producer when a cell of another chunk comes later and reads it
+                    std::vector<int32_t> pcell(cs.pos_cell);
+                    std::vector<char> producer((size_t)N, 0);
+                    for (int i = 0; i < N; ++i)
+                        for (int k = m->list_offsets[i]; k < m->list_offsets[i + 1]; ++k) {
+                            const int nbc = lnb_c[(size_t)k];
+                            if (nbc < i && chunk_of_d[(size_t)c2d[(size_t)nbc]] != chunk_of_d[(size_t)c2d[(size_t)i]])
+                                producer[(size_t)c2d[(size_t)nbc]] = 1;
+                        }
+                    for (int p = 0; p < N; ++p)
+                        if (producer[(size_t)pcell[(size_t)p]]) pcell[(size_t)p] |= kProducer;
+                    c->ot.cell = dev_upload(c.get(), pcell.data(), pcell.size());
+                } else {
+                    c->ot.cell = c->level_cells;
+                }
                 c->ot.vol = dev_upload(c.get(), ovol.data(), ovol.size());
                 c->ot.nb = dev_upload(c.get(), onb.data(), onb.size());
                 c->ot.fn = dev_upload(c.get(), ofn.data(), ofn.size());
@@ -767,8 +855,10 @@ psst_status psst_ctx_create(const psst_mesh_desc *m, const psst_settings *s, pss
     return create_impl(m, s, {}, out);
 }
 
-psst_status psst_debug_sweep_schedule(const psst_mesh_desc *m, int32_t chunk_cells, int64_t *stats) {
-    if (!m || !stats || m->n_cells <= 0 || m->n_faces <= 0 || (m->dim != 2 && m->dim != 3)) {
+psst_status psst_debug_sweep_schedule(const psst_mesh_desc *m, int32_t chunk_cells, int32_t partition,
+                                      int64_t *stats) {
+    if (!m || !stats || m->n_cells <= 0 || m->n_faces <= 0 || (m->dim != 2 && m->dim != 3) || partition < 0 ||
+        partition > 2) {
         g_create_error = "psst_debug_sweep_schedule: bad argument";
         return PSST_BAD_ARG;
     }
@@ -790,11 +880,16 @@ psst_status psst_debug_sweep_schedule(const psst_mesh_desc *m, int32_t chunk_cel
         for (int d = 0; d < N; ++d) c2d[(size_t)d2c[(size_t)d]] = d;
         const std::vector<int32_t> lev = psst::ordered_levels(N, m->list_offsets, lnb.data());
         const int32_t k = chunk_cells > 0 ? chunk_cells : sweep_chunk_cells(N, 0);
-        const psst::ChunkSchedule cs = psst::chunk_schedule(N, k, lev, m->list_offsets, lnb.data(), c2d, d2c);
+        int used = 0;
+        const psst::ChunkSchedule cs = plan_sweep(m, k, partition, 0, lev, lnb.data(), c2d, d2c, &used);
+        std::vector<int32_t> chunk_of((size_t)N);
+        for (int w = 0; w < cs.n_chunks; ++w)
+            for (int p = cs.ent_pos[(size_t)cs.ch_ent[(size_t)w]]; p < cs.ent_pos[(size_t)cs.ch_ent[(size_t)w + 1]]; ++p)
+                chunk_of[(size_t)d2c[(size_t)cs.pos_cell[(size_t)p]]] = w;
         const psst::ScheduleStats st =
-            psst::check_schedule(cs, N, lev, m->list_offsets, lnb.data(), c2d, d2c, kBlock);
-        const int64_t out[9] = {st.levels,       st.chunks,        st.entries,        st.waits, st.publishes,
-                                st.widest_entry, st.makespan_free, st.makespan_wait4, (int64_t)k};
+            psst::check_schedule(cs, N, chunk_of, lev, m->list_offsets, lnb.data(), c2d, d2c, kBlock);
+        const int64_t out[10] = {st.levels,       st.chunks,        st.entries,        st.waits,   st.publishes,
+                                 st.widest_entry, st.makespan_free, st.makespan_wait4, (int64_t)k, (int64_t)used};
         std::memcpy(stats, out, sizeof(out));
         return PSST_OK;
     } catch (const std::exception &e) {

@@ -787,20 +787,25 @@ __global__ void __launch_bounds__(1024) k_update_levels_block(Geom g, State<T> s
 
 // Tables of the ordered sweep on large meshes, by position in the level
 // order (so a level's loads are contiguous): the cell, its volume, and its
-// face list as fixed-width entries (neighbour device id, with bit 30 set
-// when the neighbour comes earlier in the reference's update loop; -1 past
-// the end), face normals and areas.  R holds each cell's residual.
+// face list as fixed-width entries (neighbour device id, with kEarlier set
+// when the neighbour comes earlier in the reference's update loop and
+// kRemote when it is also in another chunk; -1 past the end), face normals
+// and areas.  R holds each cell's residual.
 struct OrderedTables {
-    const int *__restrict__ cell;   // [N]
+    const int *__restrict__ cell;   // [N] device cell, kProducer set when another chunk reads it
     const double *__restrict__ vol; // [N]
     const int *__restrict__ nb;     // [W][N]
     const double *__restrict__ fn;  // [DIM][W][N]
     const double *__restrict__ fa;  // [W][N]
     void *R;                        // [DIM+2][N], state precision
-    int W;                          // at most kOrderedW
+    int W;                          // 4 or kOrderedW: the kernels' table width (unused rows: -1, 0)
 };
+
 constexpr int kOrderedW = 8;
-constexpr int kEarlier = 1 << 30;
+constexpr int kEarlier = 1 << 30;   // nb: the neighbour comes earlier in the update loop
+constexpr int kRemote = 1 << 29;    // nb: ... and belongs to another chunk of the chunked sweep
+constexpr int kCellMask = kRemote - 1;
+constexpr int kProducer = 1 << 30;  // cell: another chunk reads its new state
 
 // Pre-pass of the ordered sweep: every cell's residual (the ordered gather
 // of update_cell) at its position in the sweep's tables, and the residual
@@ -818,7 +823,7 @@ __global__ void __launch_bounds__(kBlock) k_ordered_residual(Geom g, State<T> s,
 #pragma unroll
     for (int j = 0; j < WS; ++j) rmax[j] = 0;
     if (p < g.N) {
-        const int c = ot.cell[p];
+        const int c = ot.cell[p] & kCellMask;
         T R[WS];
 #pragma unroll
         for (int j = 0; j < WS; ++j) R[j] = T(0);
@@ -859,6 +864,7 @@ struct OrderedHead {
 template <int DIM, typename T, int NW>
 struct OrderedIn {
     int c;
+    bool producer;
     T R[DIM + 2], w0[DIM + 2], uc[DIM], Tc, V;
     int nbw[NW];
     double fnw[NW][DIM], faw[NW];
@@ -867,16 +873,23 @@ struct OrderedIn {
 
 template <int NW>
 __device__ __forceinline__ void ordered_head(const OrderedTables &ot, int N, int p, OrderedHead<NW> &h) {
-    h.c = ot.cell[p];
+    h.c = ot.cell[p] & kCellMask;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) h.nbw[w] = w < ot.W ? ot.nb[(size_t)w * N + p] : -1;
+    for (int w = 0; w < NW; ++w) h.nbw[w] = ot.nb[(size_t)w * N + p];
 }
 
 template <int DIM, typename T, int NW>
 __device__ __forceinline__ void ordered_fetch(const Geom &g, const State<T> &s, const OrderedTables &ot, int p,
                                               const OrderedHead<NW> &h, OrderedIn<DIM, T, NW> &o) {
     constexpr int WS = DIM + 2;
-    o.c = h.c;
+    // the cell and its neighbours are loaded again (cache hits) rather than
+    // copied from the head: the head's registers are reloaded for the next
+    // level while these are still in use, and a copy would have to wait for
+    // that reload
+    const int craw = ot.cell[p];
+    o.c = craw & kCellMask;
+    o.producer = (craw & kProducer) != 0;
+
     const T *Rp = static_cast<const T *>(ot.R);
 #pragma unroll
     for (int j = 0; j < WS; ++j) {
@@ -889,11 +902,10 @@ __device__ __forceinline__ void ordered_fetch(const Geom &g, const State<T> &s, 
     o.V = T(ot.vol[p]);
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
-        const bool in = w < ot.W;
-        o.nbw[w] = h.nbw[w];
-        o.faw[w] = in ? ot.fa[(size_t)w * g.N + p] : 0.0;
+        o.nbw[w] = ot.nb[(size_t)w * g.N + p];
+        o.faw[w] = ot.fa[(size_t)w * g.N + p];
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) o.fnw[w][k] = in ? ot.fn[((size_t)k * ot.W + w) * g.N + p] : 0.0;
+        for (int k = 0; k < DIM; ++k) o.fnw[w][k] = ot.fn[((size_t)k * NW + w) * g.N + p];
         // a later neighbour (ghosts included) keeps its stage-start state
         // through the sweep; padding (-1) and earlier neighbours read cell 0
         const int nb = (h.nbw[w] < 0 || (h.nbw[w] & kEarlier)) ? 0 : h.nbw[w];
@@ -903,10 +915,11 @@ __device__ __forceinline__ void ordered_fetch(const Geom &g, const State<T> &s, 
     }
 }
 
-// Loads and stores of the new state in the chunked sweep, where workgroups
-// on different XCDs read each other's cells: agent-scope (sc1) accesses,
-// which go past the CU's L1 and keep no dirty line in the XCD's L2, so no
-// cache write-back or invalidation is needed between chunks.
+// Loads and stores of the new state that workgroups of the chunked sweep on
+// other XCDs read: agent-scope (sc1) accesses, which go past the CU's L1 and
+// the XCD's L2, so no cache write-back or invalidation is needed between
+// chunks.  A chunk's own workgroup reads its cells with plain loads (a CU
+// sees its own stores in order).
 template <bool COH, typename T>
 __device__ __forceinline__ T qn_load(const T *p) {
     if (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -918,26 +931,57 @@ __device__ __forceinline__ void qn_store(T *p, T v) {
     else *p = v;
 }
 
+// The earlier neighbours' new state of one cell, as loaded by
+// ordered_load_new: plain loads, or agent-scope loads for a neighbour in
+// another chunk.
+template <int DIM, typename T, int NW>
+struct OrderedNew {
+    T qn[NW][DIM + 1];  // (u, T) per neighbour slot
+};
+
+// Issues the loads of the new state of the cell's earlier neighbours
+// (ordered_finish uses them).  Each kind of load is masked to the lanes that
+// need it and writes registers of its own, and the choice is made after:
+// loads of different kinds into the same registers on diverged lanes would
+// make the compiler wait for every load in flight between them.  A wave
+// with no lane needing a kind skips its loads.
 template <int DIM, typename T, int NW, bool COH>
-__device__ __forceinline__ void ordered_apply(const Geom &g, const State<T> &s, const Phys &ph, T alpha,
-                                              int last_stage, const OrderedIn<DIM, T, NW> &o) {
+__device__ __forceinline__ void ordered_load_new(const Geom &g, const State<T> &s, const OrderedIn<DIM, T, NW> &o,
+                                                 OrderedNew<DIM, T, NW> &nw) {
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        const int e = o.nbw[w];
+        const bool early = e >= 0 && (e & kEarlier);
+        const bool remote = COH && early && (e & kRemote);
+        const int nb = e & kCellMask;
+        T gl[DIM + 1], rn[DIM + 1];
+        if (early && !remote)
+#pragma unroll
+            for (int k = 0; k <= DIM; ++k) gl[k] = s.qn[(size_t)(k < DIM ? k : DIM + 1) * g.NT + nb];
+        if (remote)
+#pragma unroll
+            for (int k = 0; k <= DIM; ++k) rn[k] = qn_load<true>(s.qn + (size_t)(k < DIM ? k : DIM + 1) * g.NT + nb);
+#pragma unroll
+        for (int k = 0; k <= DIM; ++k) nw.qn[w][k] = remote ? rn[k] : gl[k];
+    }
+}
+
+// The update of one cell from its loaded inputs: neighbour states (the new
+// one for neighbours earlier in the update loop, the stage-start one loaded
+// ahead otherwise), the time step, the conservative and primitive update.
+template <int DIM, typename T, int NW, bool COH>
+__device__ __forceinline__ void ordered_finish(const Geom &g, const State<T> &s, const Phys &ph, T alpha,
+                                               int last_stage, const OrderedIn<DIM, T, NW> &o,
+                                               const OrderedNew<DIM, T, NW> &nw) {
     constexpr int WS = DIM + 2;
     const int c = o.c;
-    // neighbour states: the new one for neighbours earlier in the update
-    // loop, the stage-start one (loaded ahead) otherwise
     T un[NW][DIM], Tn[NW];
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
-        if (o.nbw[w] >= 0 && (o.nbw[w] & kEarlier)) {
-            const int nb = o.nbw[w] & (kEarlier - 1);
+        const bool early = o.nbw[w] >= 0 && (o.nbw[w] & kEarlier);
 #pragma unroll
-            for (int k = 0; k < DIM; ++k) un[w][k] = qn_load<COH>(s.qn + (size_t)k * g.NT + nb);
-            Tn[w] = qn_load<COH>(s.qn + (size_t)(DIM + 1) * g.NT + nb);
-        } else {
-#pragma unroll
-            for (int k = 0; k < DIM; ++k) un[w][k] = o.un[w][k];
-            Tn[w] = o.Tn[w];
-        }
+        for (int k = 0; k < DIM; ++k) un[w][k] = early ? nw.qn[w][k] : o.un[w][k];
+        Tn[w] = early ? nw.qn[w][DIM] : o.Tn[w];
     }
     const T gr = T(ph.gamma) * T(ph.rm);
     T cc = dsqrt(gr * o.Tc);
@@ -965,17 +1009,33 @@ __device__ __forceinline__ void ordered_apply(const Geom &g, const State<T> &s, 
 #pragma unroll
         for (int j = 0; j < WS; ++j) s.W0[(size_t)j * g.N + c] = wn[j];
     const T rho = wn[DIM];
-    T vsq = T(0);
+    T vsq = T(0), qv[DIM + 3];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
         T u = fdiv(wn[k], rho);
-        qn_store<COH>(s.qn + (size_t)k * g.NT + c, u);
+        qv[k] = u;
         vsq += u * u;
     }
     T Tnew = fdiv(fdiv(wn[DIM + 1], rho) - (vsq * T(0.5)), T(ph.cv));
-    qn_store<COH>(s.qn + (size_t)DIM * g.NT + c, rho * T(ph.rm) * Tnew);
-    qn_store<COH>(s.qn + (size_t)(DIM + 1) * g.NT + c, Tnew);
-    qn_store<COH>(s.qn + (size_t)(DIM + 2) * g.NT + c, rho);
+    qv[DIM] = rho * T(ph.rm) * Tnew;
+    qv[DIM + 1] = Tnew;
+    qv[DIM + 2] = rho;
+    // a cell another chunk reads goes to memory with agent-scope stores;
+    // then every cell is stored plainly, which leaves the line in the XCD's
+    // L2 for the chunk's own next levels
+    if (COH && o.producer)
+#pragma unroll
+        for (int k = 0; k < DIM + 3; ++k) qn_store<true>(s.qn + (size_t)k * g.NT + c, qv[k]);
+#pragma unroll
+    for (int k = 0; k < DIM + 3; ++k) s.qn[(size_t)k * g.NT + c] = qv[k];
+}
+
+template <int DIM, typename T, int NW, bool COH>
+__device__ __forceinline__ void ordered_apply(const Geom &g, const State<T> &s, const Phys &ph, T alpha,
+                                              int last_stage, const OrderedIn<DIM, T, NW> &o) {
+    OrderedNew<DIM, T, NW> nw;
+    ordered_load_new<DIM, T, NW, COH>(g, s, o, nw);
+    ordered_finish<DIM, T, NW, COH>(g, s, ph, alpha, last_stage, o, nw);
 }
 
 // All three steps of one position at once (cells beyond a workgroup's first
@@ -995,7 +1055,10 @@ __device__ __forceinline__ void ordered_update(const Geom &g, const State<T> &s,
 // cells of level e are updated, the thread's position of level e + 1 is
 // fetched (its head was loaded one level earlier) and the head of its
 // position of level e + 2 is loaded.  lev_pos(e) is the first position of
-// level e of the sequence (e = n_lev: one past the last).
+// level e of the sequence (e = n_lev: one past the last).  Every thread
+// loads at every step, a thread without a position of that level at the
+// level's last position, so the number of loads in flight never depends on
+// the data and the compiler's waits before the next uses are exact.
 template <int DIM, typename T, int NW>
 struct SweepPipe {
     OrderedIn<DIM, T, NW> o;  // level e
@@ -1004,38 +1067,30 @@ struct SweepPipe {
     int ph = 0;               // position of the head
 
     template <typename LevPos>
-    __device__ __forceinline__ void start(const Geom &g, const State<T> &s, const OrderedTables &ot, int n_lev,
-                                          int tid, LevPos lev_pos) {
-        if (n_lev > 0) {
-            const int p = lev_pos(0) + tid;
-            have = p < lev_pos(1);
-            if (have) {
-                OrderedHead<NW> h0;
-                ordered_head<NW>(ot, g.N, p, h0);
-                ordered_fetch<DIM, T, NW>(g, s, ot, p, h0, o);
-            }
-        }
-        load_head(ot, g.N, 1, n_lev, tid, lev_pos);
+    __device__ __forceinline__ static int pos(int e, int n_lev, int tid, bool &valid, LevPos lev_pos) {
+        const int ec = e < n_lev ? e : n_lev - 1;
+        const int b = lev_pos(ec), en = lev_pos(ec + 1);
+        valid = e < n_lev && b + tid < en;
+        return b + tid < en ? b + tid : en - 1;
     }
     template <typename LevPos>
-    __device__ __forceinline__ void load_head(const OrderedTables &ot, int N, int e, int n_lev, int tid,
-                                              LevPos lev_pos) {
-        have_h = false;
-        if (e < n_lev) {
-            ph = lev_pos(e) + tid;
-            if (ph < lev_pos(e + 1)) {
-                ordered_head<NW>(ot, N, ph, h);
-                have_h = true;
-            }
-        }
+    __device__ __forceinline__ void start(const Geom &g, const State<T> &s, const OrderedTables &ot, int n_lev,
+                                          int tid, LevPos lev_pos) {
+        const int p = pos(0, n_lev, tid, have, lev_pos);
+        OrderedHead<NW> h0;
+        ordered_head<NW>(ot, g.N, p, h0);
+        ordered_fetch<DIM, T, NW>(g, s, ot, p, h0, o);
+        ph = pos(1, n_lev, tid, have_h, lev_pos);
+        ordered_head<NW>(ot, g.N, ph, h);
     }
     // after level e's updates: fetch level e + 1, load the head of e + 2
     template <typename LevPos>
     __device__ __forceinline__ void advance(const Geom &g, const State<T> &s, const OrderedTables &ot, int e,
                                             int n_lev, int tid, LevPos lev_pos) {
         have = have_h;
-        if (have) ordered_fetch<DIM, T, NW>(g, s, ot, ph, h, o);
-        load_head(ot, g.N, e + 2, n_lev, tid, lev_pos);
+        ordered_fetch<DIM, T, NW>(g, s, ot, ph, h, o);
+        ph = pos(e + 2, n_lev, tid, have_h, lev_pos);
+        ordered_head<NW>(ot, g.N, ph, h);
     }
 };
 
@@ -1045,23 +1100,42 @@ struct SweepPipe {
 // OrderedTables).  Before an entry it waits until every chunk
 // wait_chunk[i], i in [ent_wait[e], ent_wait[e+1]), has finished level
 // wait_level[i]; after an entry whose ent_level carries kChunkPublish it
-// publishes progress = level + 1.
+// publishes progress = level + 1.  A chunk has at most kChunkMaxEntries
+// entries and kChunkMaxWaits waits (the host checks).
 struct ChunkTables {
     const int *__restrict__ ch_ent;      // [n_chunks+1]
     const int *__restrict__ ent_pos;     // [E+1]
     const int *__restrict__ ent_level;   // [E] level | kChunkPublish
     const int *__restrict__ ent_wait;    // [E+1]
-    const int *__restrict__ wait_chunk;  // [waits]
-    const int *__restrict__ wait_level;  // [waits]
+    const int2 *__restrict__ waits;      // [waits] (chunk, level)
     int *progress;                       // [n_chunks]: 1 + the last level published, 0 at the start of a stage
     int *timed_out;                      // set to 1 when a wait gives up; then every wait gives up
     int n_chunks;
 };
 constexpr int kChunkPublish = 1 << 30;
+constexpr int kChunkMaxEntries = 1024;
+constexpr int kChunkMaxWaits = 1024;
 // Polls of one wait before it gives up (a second or two): a wait ends once
 // the chunk it waits for publishes, which the schedule guarantees; the bound
 // only keeps a broken schedule from holding the device.  The run then fails.
 constexpr int kChunkSpinLimit = 1 << 21;
+
+__device__ __forceinline__ int progress_load(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Spins until *flag >= need; gives up (and makes every other wait give up)
+// after kChunkSpinLimit polls.
+__device__ __forceinline__ void progress_wait(const int *flag, int need, int *timed_out) {
+    for (int spins = 0; progress_load(flag) < need; ++spins) {
+        if ((spins & 1023) == 1023 && progress_load(timed_out)) break;
+        if (spins > kChunkSpinLimit) {
+            __hip_atomic_store(timed_out, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
 
 // The ordered sweep as one cooperative launch of one workgroup per chunk
 // (all resident, so every wait ends).  A chunk's levels are separated by a
@@ -1069,47 +1143,60 @@ constexpr int kChunkSpinLimit = 1 << 21;
 // chunks the new state goes through agent-scope loads and stores: before a
 // publishing barrier every wave waits until its stores are complete
 // (s_waitcnt vmcnt(0)), so the progress word thread 0 then stores is seen
-// only after the cells it covers.  The entries are pipelined (SweepPipe),
-// so a level's chain is only the earlier neighbours' new state and the
-// update.
+// only after the cells it covers.
+// The chunk's entry tables are staged in LDS (vector loads of them would
+// make every wait on them also wait for the prefetches in flight).  The
+// cell loads are pipelined (SweepPipe), and the first wave reads the
+// progress words an entry ahead, before that entry's prefetches: when the
+// waited-for chunks are ahead, the check finds the words already read and
+// the level's chain is only the earlier neighbours' new state and the update.
 template <int DIM, typename T, int NW>
 __global__ void __launch_bounds__(kBlock) k_update_chunks(Geom g, State<T> s, Phys ph, T alpha, int last_stage,
                                                           OrderedTables ot, ChunkTables ct) {
+    __shared__ int s_pos[kChunkMaxEntries + 1], s_lev[kChunkMaxEntries], s_wait[kChunkMaxEntries + 1];
+    __shared__ int2 s_waits[kChunkMaxWaits];
     if (run_stopped(g)) return;
     const int w = (int)blockIdx.x, tid = (int)threadIdx.x, nthr = (int)blockDim.x;
-    const int e0 = ct.ch_ent[w], e1 = ct.ch_ent[w + 1];
-    const int *ent_pos = ct.ent_pos + e0;
-    auto lev_pos = [ent_pos](int e) { return ent_pos[e]; };
+    const int e0 = ct.ch_ent[w], n_ent = ct.ch_ent[w + 1] - e0;
+    const int w0 = ct.ent_wait[e0], n_waits = ct.ent_wait[e0 + n_ent] - w0;
+    for (int i = tid; i <= n_ent; i += nthr) {
+        s_pos[i] = ct.ent_pos[e0 + i];
+        s_wait[i] = ct.ent_wait[e0 + i] - w0;
+        if (i < n_ent) s_lev[i] = ct.ent_level[e0 + i];
+    }
+    for (int i = tid; i < n_waits; i += nthr) s_waits[i] = ct.waits[w0 + i];
+    __syncthreads();
+    auto lev_pos = [](int e) { return s_pos[e]; };
+    // first wave: lane i reads the progress word of wait i of the next entry
+    // (lanes without one read the chunk's own word and need nothing)
+    const bool poller = tid < 64;
+    int poll_v = 0, poll_need = 0;
+    auto poll_ahead = [&](int e) {
+        if (!poller) return;
+        const int wb = e < n_ent ? s_wait[e] : 0, nw = e < n_ent ? s_wait[e + 1] - wb : 0;
+        const int2 wt = tid < nw ? s_waits[wb + tid] : make_int2(w, -1);
+        poll_need = wt.y + 1;
+        poll_v = progress_load(ct.progress + wt.x);
+    };
+    poll_ahead(0);
     SweepPipe<DIM, T, NW> pipe;
-    pipe.start(g, s, ot, e1 - e0, tid, lev_pos);
-    for (int e = e0; e < e1; ++e) {
-        const int wb = ct.ent_wait[e], we = ct.ent_wait[e + 1];
-        if (wb < we) {
-            for (int i = wb + tid; i < we; i += nthr) {
-                const int need = ct.wait_level[i] + 1;
-                const int *flag = ct.progress + ct.wait_chunk[i];
-                for (int spins = 0; __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need;
-                     ++spins) {
-                    // once any wait has given up, every other one gives up at once
-                    if ((spins & 1023) == 1023 &&
-                        __hip_atomic_load(ct.timed_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                        break;
-                    if (spins > kChunkSpinLimit) {
-                        __hip_atomic_store(ct.timed_out, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-            }
+    pipe.start(g, s, ot, n_ent, tid, lev_pos);
+    for (int e = 0; e < n_ent; ++e) {
+        const int wb = s_wait[e], nw = s_wait[e + 1] - wb;
+        if (nw > 0) {
+            if (poller && poll_v < poll_need) progress_wait(ct.progress + s_waits[wb + tid].x, poll_need, ct.timed_out);
+            for (int i = 64 + tid; i < nw; i += nthr)  // more waits than one wave (rare)
+                progress_wait(ct.progress + s_waits[wb + i].x, s_waits[wb + i].y + 1, ct.timed_out);
             __syncthreads();
         }
-        const int b = ct.ent_pos[e], en = ct.ent_pos[e + 1];
+        const int b = s_pos[e], en = s_pos[e + 1];
+        const int lv = s_lev[e];
         if (pipe.have) ordered_apply<DIM, T, NW, true>(g, s, ph, alpha, last_stage, pipe.o);
         for (int p = b + tid + nthr; p < en; p += nthr)
             ordered_update<DIM, T, NW, true>(g, s, ph, alpha, last_stage, ot, p);
-        const int lv = ct.ent_level[e];
         if (lv & kChunkPublish) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        pipe.advance(g, s, ot, e - e0, e1 - e0, tid, lev_pos);
+        poll_ahead(e + 1);
+        pipe.advance(g, s, ot, e, n_ent, tid, lev_pos);
         __syncthreads();
         if ((lv & kChunkPublish) && tid == 0)
             __hip_atomic_store(ct.progress + w, (lv & (kChunkPublish - 1)) + 1, __ATOMIC_RELAXED,
