@@ -50,8 +50,9 @@ void axis_spacing(const psst_mesh_desc *m, double axis_h[3]) {
 // Cells per chunk of the chunked ordered sweep: PSST_SWEEP_CHUNK_CELLS if
 // set (a tuning knob), else 8,192; at least N / capacity, so that the chunks
 // fit one cooperative launch (capacity 0: no limit known).  On the
-// 4.9M-cell wedge 9,600 cells per chunk (the capacity bound) sweep a stage
-// in 15.7 ms, 16,384 in 16.5 ms and 32,768 in 19.1 ms.
+// 4.9M-cell wedge (capacity 512): 481 boxes of the potentials sweep a stage
+// in 15.2 ms, 300 in 16.5 ms, 150 in 19.1 ms; at three waves per SIMD
+// (capacity 768, a small spill), 645 boxes took 16.6 ms.
 int32_t sweep_chunk_cells(int32_t N, int capacity) {
     int32_t k = 8192;
     if (const char *e = std::getenv("PSST_SWEEP_CHUNK_CELLS")) {
@@ -85,9 +86,18 @@ psst::ChunkSchedule plan_sweep(const psst_mesh_desc *m, int32_t chunk_cells, int
     for (int part : {1, 2}) {
         if (partition != 0 && partition != part) continue;
         std::vector<int32_t> chunk_of;
-        const int32_t n = part == 1 ? psst::morton_chunks(N, chunk_cells, c2d, chunk_of)
-                                    : psst::potential_chunks(N, m->dim, chunk_cells, m->list_offsets, m->list_faces,
-                                                             lnb, m->face_normal, m->n_faces, chunk_of);
+        int32_t n = 0;
+        if (part == 1) {
+            n = psst::morton_chunks(N, chunk_cells, c2d, chunk_of);
+        } else {
+            // boxes whose count exceeds the limit (the domain is not a whole number of
+            // boxes) grow until it fits
+            for (int32_t k = chunk_cells; k <= N; k = k + k / 8 + 1) {
+                n = psst::potential_chunks(N, m->dim, k, m->list_offsets, m->list_faces, lnb, m->face_normal,
+                                           m->n_faces, chunk_of);
+                if (max_chunks <= 0 || n <= max_chunks) break;
+            }
+        }
         if (partition == 0 && max_chunks > 0 && n > max_chunks) continue;
         // a publication stride above 1 can close a cycle of waits between chunks that
         // wait for each other; such a schedule publishes after every waited-for level
@@ -595,9 +605,11 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                         chunked = e1 - e0 <= kChunkMaxEntries &&
                                   cs.ent_wait[(size_t)e1] - cs.ent_wait[(size_t)e0] <= kChunkMaxWaits;
                     }
-                    psst::log_phase(!chunked ? "context: chunk tables too large, ordered sweep by levels"
-                                    : used == 1 ? "context: ordered sweep by Morton chunks"
-                                                : "context: ordered sweep by potential boxes");
+                    const std::string msg =
+                        !chunked ? std::string("context: chunk tables too large, ordered sweep by levels")
+                                 : "context: ordered sweep by " + std::to_string(cs.n_chunks) +
+                                       (used == 1 ? " Morton chunks" : " potential boxes");
+                    psst::log_phase(msg.c_str());
                 }
                 if (chunked) {
                     chunk_of_d.resize((size_t)N);
