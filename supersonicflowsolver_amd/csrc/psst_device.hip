@@ -92,10 +92,10 @@ psst::ChunkSchedule plan_sweep(const psst_mesh_desc *m, int32_t chunk_cells, int
         } else {
             // boxes whose count exceeds the limit (the domain is not a whole number of
             // boxes) grow until it fits
-            for (int32_t k = chunk_cells; k <= N; k = k + k / 8 + 1) {
+            for (int32_t k = chunk_cells;; k = k + k / 8 + 1) {
                 n = psst::potential_chunks(N, m->dim, k, m->list_offsets, m->list_faces, lnb, m->face_normal,
                                            m->n_faces, chunk_of);
-                if (max_chunks <= 0 || n <= max_chunks) break;
+                if (max_chunks <= 0 || n <= max_chunks || k >= N) break;
             }
         }
         if (partition == 0 && max_chunks > 0 && n > max_chunks) continue;
