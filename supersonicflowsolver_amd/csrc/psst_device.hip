@@ -63,13 +63,15 @@ int32_t sweep_chunk_cells(int32_t N, int capacity) {
 }
 
 // Levels between publications of a chunk's progress:
-// PSST_SWEEP_PUBLISH_STRIDE if set (a tuning knob), else 4.
+// PSST_SWEEP_PUBLISH_STRIDE if set (a tuning knob), else 1 (after every
+// waited-for level).  On the 4.9M-cell wedge strides 1, 2, 4, 8 and 16 gave
+// 59.2, 60.5, 59.7, 61.5 and 65.7 ms per iteration.
 int32_t sweep_publish_stride() {
     if (const char *e = std::getenv("PSST_SWEEP_PUBLISH_STRIDE")) {
         const long v = std::strtol(e, nullptr, 10);
         if (v > 0 && v <= 1024) return (int32_t)v;
     }
-    return 4;
+    return 1;
 }
 
 // Chunk schedule of the reference-ordered sweep with chunks of about
@@ -586,12 +588,17 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                 // Large meshes with wide levels sweep by chunks (one workgroup
                 // per chunk, waits only between chunks that share faces); the
                 // tables are then in chunk order, else in level order.
-                int capacity = 0;
-                if (!(widest <= 256 || N <= 65536)) {
-                    const bool f64 = s->precision == 64;
-                    capacity = dim == 2 ? (f64 ? chunk_sweep_capacity<2, double>(W) : chunk_sweep_capacity<2, float>(W))
-                                        : (f64 ? chunk_sweep_capacity<3, double>(W) : chunk_sweep_capacity<3, float>(W));
-                }
+                // workgroups of 128 threads when no level of a chunk is wider (twice
+                // the chunks fit one cooperative launch), else 256
+                const bool f64 = s->precision == 64;
+                auto capacity_of = [&](int threads) {
+                    return dim == 2 ? (f64 ? chunk_sweep_capacity<2, double>(W, threads)
+                                           : chunk_sweep_capacity<2, float>(W, threads))
+                                    : (f64 ? chunk_sweep_capacity<3, double>(W, threads)
+                                           : chunk_sweep_capacity<3, float>(W, threads));
+                };
+                const bool large = !(widest <= 256 || N <= 65536);
+                int threads = 128, capacity = large ? capacity_of(threads) : 0;
                 bool chunked = capacity > 0 && NT < kRemote;
                 psst::ChunkSchedule cs;
                 std::vector<int32_t> chunk_of_d;  // chunk of every device interior cell
@@ -599,6 +606,16 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                     int used = 0;
                     cs = plan_sweep(m, sweep_chunk_cells(N, capacity), 0, capacity, lev, lnb_c.data(), c2d, d2c,
                                     &used);
+                    int wide = 0;
+                    for (size_t e = 0; e + 1 < cs.ent_pos.size(); ++e) wide = std::max(wide, cs.ent_pos[e + 1] - cs.ent_pos[e]);
+                    if (wide > threads) {
+                        threads = kBlock;
+                        capacity = capacity_of(threads);
+                        chunked = capacity > 0;
+                        if (chunked)
+                            cs = plan_sweep(m, sweep_chunk_cells(N, capacity), 0, capacity, lev, lnb_c.data(), c2d,
+                                            d2c, &used);
+                    }
                     // a chunk's entry tables must fit the kernel's LDS
                     for (int w = 0; w < cs.n_chunks && chunked; ++w) {
                         const int32_t e0 = cs.ch_ent[(size_t)w], e1 = cs.ch_ent[(size_t)w + 1];
@@ -608,7 +625,8 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                     const std::string msg =
                         !chunked ? std::string("context: chunk tables too large, ordered sweep by levels")
                                  : "context: ordered sweep by " + std::to_string(cs.n_chunks) +
-                                       (used == 1 ? " Morton chunks" : " potential boxes");
+                                       (used == 1 ? " Morton chunks" : " potential boxes") + " of " +
+                                       std::to_string(threads) + " threads";
                     psst::log_phase(msg.c_str());
                 }
                 if (chunked) {
@@ -629,6 +647,7 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                     ct.timed_out = dev_alloc<int>(c.get(), 1);  // used outside psst_run's control words
                     hip_check(hipMemset(ct.timed_out, 0, sizeof(int)), "hipMemset");
                     ct.n_chunks = cs.n_chunks;
+                    ct.threads = threads;
                 }
                 const std::vector<int32_t> &order = chunked ? cs.pos_cell : cells;
                 std::vector<int> onb((size_t)W * N, -1);

@@ -1111,6 +1111,7 @@ struct ChunkTables {
     int *progress;                       // [n_chunks]: 1 + the last level published, 0 at the start of a stage
     int *timed_out;                      // set to 1 when a wait gives up; then every wait gives up
     int n_chunks;
+    int threads;                         // workgroup size: 128 when no level of a chunk is wider, else 256
 };
 constexpr int kChunkPublish = 1 << 30;
 constexpr int kChunkMaxEntries = 1024;
@@ -2347,19 +2348,20 @@ void run_flux(psst_ctx *c) {
     }
 }
 
-// Workgroups that one cooperative launch of the chunked ordered sweep holds
-// on the current device for tables of width W (0: no cooperative launches).
+// Workgroups of `threads` threads that one cooperative launch of the chunked
+// ordered sweep holds on the current device for tables of width W (0: no
+// cooperative launches).
 template <int DIM, typename T>
-int chunk_sweep_capacity(int W) {
+int chunk_sweep_capacity(int W, int threads) {
     int dev = 0, coop = 0, cus = 0, per_cu = 0;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     hip_check(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev), "hipDeviceGetAttribute");
     hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
     if (W <= 4)
-        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_update_chunks<DIM, T, 4>, kBlock, 0),
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_update_chunks<DIM, T, 4>, threads, 0),
                   "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     else
-        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_update_chunks<DIM, T, kOrderedW>, kBlock, 0),
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_update_chunks<DIM, T, kOrderedW>, threads, 0),
                   "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     return coop ? cus * per_cu : 0;
 }
@@ -2369,7 +2371,8 @@ int chunk_sweep_capacity(int W) {
 template <int DIM, typename T>
 bool chunk_sweep_fits(psst_ctx *c) {
     int &ok = c->chunk_ok[sizeof(T) == 8 ? 1 : 0];
-    if (ok < 0) ok = (c->ct.n_chunks > 0 && c->ct.n_chunks <= chunk_sweep_capacity<DIM, T>(c->ot.W)) ? 1 : 0;
+    if (ok < 0)
+        ok = (c->ct.n_chunks > 0 && c->ct.n_chunks <= chunk_sweep_capacity<DIM, T>(c->ot.W, c->ct.threads)) ? 1 : 0;
     return ok == 1;
 }
 
@@ -2390,8 +2393,8 @@ bool run_ordered_tables(psst_ctx *c, const State<T> &st, T alpha, int last, unsi
         if (c->geom.stop) ct.timed_out = const_cast<int *>(c->geom.stop) + 3;
         launch(c, "ordered_residual", k_ordered_residual<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st,
                c->ot, res_row, ct.progress, ct.n_chunks);
-        launch_coop(c, "update_chunks", k_update_chunks<DIM, T, NW>, dim3(ct.n_chunks), dim3(kBlock), c->geom, st,
-                    c->ph, alpha, last, c->ot, ct);
+        launch_coop(c, "update_chunks", k_update_chunks<DIM, T, NW>, dim3(ct.n_chunks), dim3(ct.threads), c->geom,
+                    st, c->ph, alpha, last, c->ot, ct);
         return true;
     }
     if (!narrow) return false;
@@ -2493,4 +2496,4 @@ void run_debug_residual(psst_ctx *c, double *out) {
     EXT template void psst_dev::run_set_w<DIM, T>(psst_ctx *, int, int);                      \
     EXT template void psst_dev::run_grad<DIM, T>(psst_ctx *);                                 \
     EXT template void psst_dev::run_debug_residual<DIM, T>(psst_ctx *, double *);         \
-    EXT template int psst_dev::chunk_sweep_capacity<DIM, T>(int);
+    EXT template int psst_dev::chunk_sweep_capacity<DIM, T>(int, int);
