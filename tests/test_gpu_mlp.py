@@ -45,17 +45,27 @@ def test_forward_and_gradients_match_oracle(n):
     out_cpu = mo.forward(p, x, DIMS)
     err = np.abs(out_gpu - out_cpu).max() / max(1.0, np.abs(out_cpu).max())
     print("n=%d forward max rel err %.2e" % (n, err), flush=True)
-    assert err < 2e-3
+    # measured on MI355X: at most 7.4e-5 over the three sizes
+    assert err < 3e-4
     loss_gpu, g_gpu = net.loss_grad(x, y)
     loss_cpu, g_cpu = mo.loss_grad(p, x, y, DIMS)
-    assert abs(loss_gpu - loss_cpu) / loss_cpu < 2e-3
+    lrel = abs(loss_gpu - loss_cpu) / loss_cpu
+    print("n=%d loss rel err %.2e" % (n, lrel), flush=True)
+    assert lrel < 1e-5  # measured at most 3e-7
+    # per parameter block: measured at most 1.3e-3 (the bias blocks, summed over
+    # 64 samples) and 2.7e-4 from 1,000 samples on
+    bound = 4e-3 if n < 1000 else 1e-3
     for k, sl in enumerate(_layer_slices(DIMS)):
         a, b = g_gpu[sl].astype(np.float64), g_cpu[sl]
         rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
         print("n=%d param block %d: rel L2 %.2e" % (n, k, rel), flush=True)
-        assert rel < 1e-2, (k, rel)
+        assert rel < bound, (k, rel)
+    # against the gradient without BF16 rounding: the BF16 rounding of weights and
+    # activations itself (measured 2.7e-3 to 4.4e-3)
     _, g_exact = mo.grad_exact(p, x, y, DIMS)
-    assert np.linalg.norm(g_gpu - g_exact) / np.linalg.norm(g_exact) < 0.1
+    xrel = np.linalg.norm(g_gpu - g_exact) / np.linalg.norm(g_exact)
+    print("n=%d gradient rel L2 against the exact FP64 gradient %.2e" % (n, xrel), flush=True)
+    assert xrel < 2e-2
     net.close()
 
 
