@@ -51,7 +51,10 @@ def parse():
     ap.add_argument("--ref-iters", type=int, default=5,
                     help="iterations of the reference build's sample (a 4x coarser wedge, about 12 s; 0: skip)")
     ap.add_argument("--also-fp32", type=int, default=1, help="also time the FP32 mode")
-    ap.add_argument("--timing-iters", type=int, default=10)
+    ap.add_argument("--timing-iters", type=int, default=-1,
+                    help="-1: replay the timed region from its starting state with per-kernel HIP events (the "
+                         "kernels' mean durations over the same iterations); N > 0: N more iterations after it; "
+                         "0: no kernel timing")
     ap.add_argument("--profile", action="store_true", help="short run for rocprof (no CPU baseline, no JSON extras)")
     ap.add_argument("--tile-cells", type=int, default=0)
     ap.add_argument("--tile-threads", type=int, default=0)
@@ -638,26 +641,36 @@ def main():
         print("bench: FP%d context ready (%.1f s)" % (precision, time.perf_counter() - t), file=sys.stderr,
               flush=True)
         s.run(a.warmup)
+        start = s.state() if a.timing_iters < 0 else None
         t = time.perf_counter()
         s.run(a.steps)
         el = time.perf_counter() - t
         print("bench: FP%d timed run done (%.3f s)" % (precision, el), file=sys.stderr, flush=True)
-        # per-kernel mean durations (HIP events on the solver's stream)
-        s.kernel_timing(True)
-        s.run(a.timing_iters)
-        stats = {}
-        for name in s.kernel_names():
-            cnt, ms = s.kernel_stats(name)
-            if cnt:
-                stats[name] = (cnt, ms)
-        s.kernel_timing(False)
+        # per-kernel mean durations (HIP events on the solver's stream).  The
+        # kernels' cost follows the flow (limiter and wave-speed branches
+        # diverge more once the shock has formed), so by default the timed
+        # region is replayed from its starting state with the events on: the
+        # same iterations as `value`, without the events' cost in `value`.
+        stats, el_replay = {}, None
+        if a.timing_iters != 0:
+            if start is not None:
+                s.set_state(start)
+            s.kernel_timing(True)
+            t = time.perf_counter()
+            s.run(a.steps if start is not None else a.timing_iters)
+            el_replay = time.perf_counter() - t
+            for name in s.kernel_names():
+                cnt, ms = s.kernel_stats(name)
+                if cnt:
+                    stats[name] = (cnt, ms)
+            s.kernel_timing(False)
         state = s.state()
         s.close()
-        return el, stats, state
+        return el, stats, state, el_replay
 
     print("bench: FP%d, %d warmup + %d timed iterations" % (a.precision, a.warmup, a.steps), file=sys.stderr,
           flush=True)
-    el, stats, state = timed(a.precision)
+    el, stats, state, el_replay = timed(a.precision)
     if a.profile:
         print("bench: done (%.3f s)" % el, file=sys.stderr, flush=True)
         return
@@ -666,7 +679,8 @@ def main():
     b = 8 if a.precision == 64 else 4
     cell_its = N * a.steps / el
     # dominant kernel by total time per iteration
-    per_iter = {k: cnt * ms / a.timing_iters for k, (cnt, ms) in stats.items()}
+    timing_iters = a.steps if a.timing_iters < 0 else a.timing_iters
+    per_iter = {k: cnt * ms / timing_iters for k, (cnt, ms) in stats.items()}
     dom = max(per_iter, key=per_iter.get)
     kind = {"flux": "flux", "update": "update", "grad_gg": "grad", "grad_lsq": "grad", "bc": "bc",
             "stage": "stage", "stage_last": "stage", "update_levels_block": "update",
@@ -678,7 +692,7 @@ def main():
     if dom in ("update_levels_block", "update_levels_wg", "update_level", "update_chunks"):
         # time step mode 1 in the reference's order: each stage's update is one sweep over the cells'
         # dependency levels (one launch, or one per level); rate over a whole stage's sweep
-        mean_ms = stats[dom][0] * stats[dom][1] / (a.timing_iters * 4)
+        mean_ms = stats[dom][0] * stats[dom][1] / (timing_iters * 4)
         note = "reference-ordered update sweep (a chain of dependency levels): latency-bound, not HBM-bound"
     achieved = nbytes / (mean_ms * 1e-3) / 1e9
     # HBM traffic of the dominant kernel: rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs (each scaled by
@@ -694,7 +708,11 @@ def main():
             traffic = tr["bytes_per_launch"]
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "alg_bytes_per_launch": int(nbytes), "mean_ms": round(mean_ms, 4)}
+                "alg_bytes_per_launch": int(nbytes), "mean_ms": round(mean_ms, 4),
+                "timed_over": ("a replay of the timed region (same start state, same %d iterations) with HIP events "
+                               "around every launch; %.4f ms per step with the events" %
+                               (a.steps, el_replay / a.steps * 1e3)) if a.timing_iters < 0 and el_replay else
+                              ("%d iterations after the timed region" % timing_iters)}
     if note:
         roofline["note"] = note
     whole = model_bytes_per_cell_iteration(dim, b, N, F)
@@ -719,7 +737,7 @@ def main():
         "kernels_ms": {k: round(v[1], 4) for k, v in stats.items()},
     }
     if a.also_fp32 and a.precision == 64:
-        el32, stats32, state32 = timed(32)
+        el32, stats32, state32, _ = timed(32)
         ci32 = N * a.steps / el32
         whole32 = model_bytes_per_cell_iteration(dim, 4, N, F)
         # relative L2 per field; the velocity components against the norm of

@@ -1652,6 +1652,100 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(Stage
     });
 }
 
+// Tiles of at most one cell and two slots per thread: the thread's slot
+// cells (own, or the halo table's) are found, then their q is loaded and
+// written to LDS.  `between` runs after the slot cells are known and before
+// their q loads: the Green-Gauss kernel issues its cell's face loads there,
+// so that those and the slot loads are in flight together before the
+// workgroup's barrier (two dependent round trips to memory instead of four).
+template <int NV, typename T, typename Between>
+__device__ __forceinline__ void stage_two_slots(const Geom &g, const TileDev &td, const State<T> &s, int cb, int nc,
+                                                int hb, int ns, T *sq, Between between) {
+    const int ia = threadIdx.x, ib = threadIdx.x + blockDim.x;
+    // past the tile's slots, a thread reads the tile's first cell (in bounds)
+    const int ca = ia < nc ? cb + ia : (ia < ns ? td.halo[hb + ia - nc] : cb);
+    const int cc = ib < nc ? cb + ib : (ib < ns ? td.halo[hb + ib - nc] : cb);
+    between();
+    T qa[NV], qb[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        qa[v] = s.q[(size_t)v * g.NT + ca];
+        qb[v] = s.q[(size_t)v * g.NT + cc];
+    }
+    if (ia < ns)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sq[v * td.MS + ia] = qa[v];
+    if (ib < ns)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sq[v * td.MS + ib] = qb[v];
+}
+
+// One face's Green-Gauss term for the cell in slot i (psstGradCalculation.cpp:
+// 80-177): the face value beta q_0 + (1 - beta) q_1, signed outward, times
+// n A / V.
+template <int DIM, typename T>
+__device__ __forceinline__ void gg_term(const TileDev &td, const T *sq, int i, double Vc, unsigned e, double fa,
+                                        double be, const double *fn, T (&acc)[DIM + 2][DIM]) {
+    constexpr int NV = DIM + 2;
+    const int l0 = end0(e), l1 = end1(e);
+    const T geom = T(fa / Vc);
+    const T beta = T(be);
+    const T rbeta = T(1) - beta;
+    const bool first = (l0 == i);
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+        T phi = beta * sq[n * td.MS + l0] + rbeta * sq[n * td.MS + l1];
+        T sp = first ? -phi : phi;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) acc[n][k] += sp * geom * T(fn[k]);
+    }
+}
+
+// Green-Gauss on a tile of at most one cell per thread and at most U faces
+// per cell: the cell's visit list, volume and face data are loaded with the
+// slots, before the barrier.  On the 4.9M-cell wedge (FP64): 0.244 vs
+// 0.265 ms against loading them after the barrier, bitwise unchanged.
+template <int DIM, typename T, int U>
+__device__ __forceinline__ void grad_gg_tile_prefetched(const Geom &g, const TileDev &td, const State<T> &s, T *sq,
+                                                        int cb, int nc, int hb, int ns, int fb) {
+    constexpr int NV = DIM + 2;
+    const int i = threadIdx.x;
+    const bool own = i < nc;
+    const int c = cb + (own ? i : 0);
+    // the visit list is padded with -1 to Wv <= U entries; a padding entry
+    // reads the tile's face 0 (in bounds) and is skipped
+    int ev[U];
+#pragma unroll
+    for (int w = 0; w < U; ++w) ev[w] = w < td.Wv ? (int)td.visit[(size_t)w * g.N + c] : -1;
+    const double Vc = g.vol[c];
+    unsigned ee[U];
+    double fa[U], be[U], fn[U][DIM];
+    stage_two_slots<NV, T>(g, td, s, cb, nc, hb, ns, sq, [&] {
+#pragma unroll
+        for (int w = 0; w < U; ++w) {
+            const int lf = ev[w] < 0 ? 0 : (ev[w] & 0x3fff);
+            ee[w] = td.ends[fb + lf];
+            fa[w] = td.fa[fb + lf];
+            be[w] = td.beta[fb + lf];
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) fn[w][k] = td.fn[(size_t)k * td.TF + fb + lf];
+        }
+    });
+    __syncthreads();
+    if (!own) return;
+    T acc[NV][DIM];
+#pragma unroll
+    for (int n = 0; n < NV; ++n)
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) acc[n][k] = T(0);
+#pragma unroll
+    for (int w = 0; w < U; ++w) {
+        if (ev[w] < 0) break;
+        gg_term<DIM, T>(td, sq, i, Vc, ee[w], fa[w], be[w], fn[w], acc);
+    }
+    store_grads<DIM, T>(s.grad, c, acc);
+}
+
 // Green-Gauss gradients on a tile (psstGreenGauss::calculate,
 // psstGradCalculation.cpp:80-177): q of own and halo cells in LDS, each
 // cell's faces in face-loop order.
@@ -1663,6 +1757,14 @@ __device__ __forceinline__ void grad_gg_tile_body(const Geom &g, const TileDev &
     const int cb = td.cell_begin[t], nc = td.cell_begin[t + 1] - cb;
     const int hb = td.halo_off[t], ns = nc + (td.halo_off[t + 1] - hb);
     const int fb = td.face_off[t];
+    // 2D tiles of quadrilaterals (or triangles) only: in 3D the face data of
+    // a hexahedron held across the barrier would cost occupancy (the same
+    // form of the least-squares kernel on the 12.6M-hexahedra wedge, 147
+    // VGPRs against 100, took 1.40 ms against 1.15)
+    if (DIM == 2 && nc <= (int)blockDim.x && ns <= 2 * (int)blockDim.x && td.Wv <= 4) {
+        grad_gg_tile_prefetched<DIM, T, 4>(g, td, s, sq, cb, nc, hb, ns, fb);
+        return;
+    }
     for (int i = threadIdx.x; i < ns; i += blockDim.x) {
         const int cell = i < nc ? cb + i : td.halo[hb + i - nc];
 #pragma unroll
@@ -1677,20 +1779,8 @@ __device__ __forceinline__ void grad_gg_tile_body(const Geom &g, const TileDev &
 #pragma unroll
             for (int k = 0; k < DIM; ++k) acc[n][k] = T(0);
         const double Vc = g.vol[c];
-        // one face's term, from its ends, area, beta and normal
         auto term = [&](unsigned e, double fa, double be, const double *fn) {
-            const int l0 = end0(e), l1 = end1(e);
-            const T geom = T(fa / Vc);
-            const T beta = T(be);
-            const T rbeta = T(1) - beta;
-            const bool first = (l0 == i);
-#pragma unroll
-            for (int n = 0; n < NV; ++n) {
-                T phi = beta * sq[n * td.MS + l0] + rbeta * sq[n * td.MS + l1];
-                T sp = first ? -phi : phi;
-#pragma unroll
-                for (int k = 0; k < DIM; ++k) acc[n][k] += sp * geom * T(fn[k]);
-            }
+            gg_term<DIM, T>(td, sq, i, Vc, e, fa, be, fn, acc);
         };
         auto unrolled = [&](auto width) {
             // every load of the cell's faces is issued before the first
@@ -1748,6 +1838,22 @@ __global__ void __launch_bounds__(NTHR) k_grad_gg_tile(Geom g, TileDev td, State
     });
 }
 
+// One neighbour's least-squares term for the cell in slot i
+// (psstGradCalculation.cpp:268-368): the weights w theta_k times the
+// difference of the neighbour's q and the cell's.
+template <int DIM, typename T>
+__device__ __forceinline__ void lsq_term(const TileDev &td, const T *sq, int i, unsigned e, const double *wk,
+                                         T (&acc)[DIM + 2][DIM]) {
+    constexpr int NV = DIM + 2;
+    const int nbs = end0(e) == i ? end1(e) : end0(e);
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+        T d = (sq[n * td.MS + nbs] - sq[n * td.MS + i]);
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) acc[n][k] += T(wk[k]) * d;
+    }
+}
+
 // Least-squares gradients on a tile (psstLeastSquare::calculate,
 // psstGradCalculation.cpp:268-368), face-list order.
 template <int DIM, typename T>
@@ -1771,16 +1877,7 @@ __device__ __forceinline__ void grad_lsq_tile_body(const Geom &g, const TileDev 
         for (int n = 0; n < NV; ++n)
 #pragma unroll
             for (int k = 0; k < DIM; ++k) acc[n][k] = T(0);
-        // one neighbour's term, from the face's ends and the weights w*theta_k
-        auto term = [&](unsigned e, const double *wk) {
-            const int nbs = end0(e) == i ? end1(e) : end0(e);
-#pragma unroll
-            for (int n = 0; n < NV; ++n) {
-                T d = (sq[n * td.MS + nbs] - sq[n * td.MS + i]);
-#pragma unroll
-                for (int k = 0; k < DIM; ++k) acc[n][k] += T(wk[k]) * d;
-            }
-        };
+        auto term = [&](unsigned e, const double *wk) { lsq_term<DIM, T>(td, sq, i, e, wk, acc); };
         auto unrolled = [&](auto width) {
             // every load of the cell's neighbours is issued before the first
             // term: the face list and the weights are padded to Wl entries
