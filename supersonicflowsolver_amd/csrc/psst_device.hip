@@ -780,11 +780,33 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                 cands = {768, 512, 384, 256, 192, 128};
             }
             const double cost_slot = (double)(dim + 3) * sb + sizeof(int), cost_face = (double)(dim + 2) * sb;
+            // 2D: a tile's faces grouped by the axis their normal is closest
+            // to, boundary faces last.  The lanes of a wavefront of the
+            // gradient kernel then read the same face of neighbouring cells
+            // at neighbouring addresses.  On the 4.9M-cell wedge: Green-Gauss
+            // 0.235 vs 0.243 ms (FP64), 0.158 vs 0.176 ms (FP32); the stage
+            // kernel 0.372 vs 0.370 ms (FP64), 0.256 vs 0.258 ms (FP32); all
+            // bitwise unchanged.  On the 12.6M-hexahedra wedge the 3D stage
+            // kernel took 2.40 vs 2.05 ms, so 3D keeps the order of first
+            // appearance over the tile's cells.
+            std::vector<uint8_t> fclass(dim == 2 ? (size_t)F : 0);
+            for (int f = 0; f < (dim == 2 ? F : 0); ++f) {
+                if (f0d[(size_t)f] >= N || f1d[(size_t)f] >= N) {
+                    fclass[(size_t)f] = (uint8_t)dim;
+                    continue;
+                }
+                int best = 0;
+                for (int k = 1; k < dim; ++k)
+                    if (std::fabs(m->face_normal[(size_t)k * F + f]) > std::fabs(m->face_normal[(size_t)best * F + f]))
+                        best = k;
+                fclass[(size_t)f] = (uint8_t)best;
+            }
             psst::TilePlan tp;
             size_t shm_stage = 0, shm_grad = 0;
             for (size_t ci = 0; ci < cands.size(); ++ci) {
                 tp = psst::plan_tiles(N, G, F, cands[ci], f0d.data(), f1d.data(), voff.data(), vface.data(),
-                                      loff.data(), lface.data(), seg, cost_slot, cost_face);
+                                      loff.data(), lface.data(), seg, cost_slot, cost_face,
+                                      fclass.empty() ? nullptr : fclass.data());
                 c->max_slots = tp.max_slots;
                 c->max_faces = tp.max_faces;
                 shm_stage = stage_shm(c.get(), sb);

@@ -78,8 +78,8 @@ namespace {
 
 // Slots (own cells + halo) and faces of the tile [b, e).
 void tile_size(int32_t b, int32_t e, const int32_t *f0, const int32_t *f1, const int32_t *voff, const int32_t *vface,
-               std::vector<int32_t> &slot_of, std::vector<int32_t> &lface_of, std::vector<int32_t> &tile_faces,
-               std::vector<int32_t> &halo_cells) {
+               const uint8_t *face_class, std::vector<int32_t> &slot_of, std::vector<int32_t> &lface_of,
+               std::vector<int32_t> &tile_faces, std::vector<int32_t> &halo_cells) {
     const int32_t nc = e - b;
     for (int32_t c = b; c < e; ++c) slot_of[(size_t)c] = c - b;
     tile_faces.clear();
@@ -92,6 +92,13 @@ void tile_size(int32_t b, int32_t e, const int32_t *f0, const int32_t *f1, const
                 tile_faces.push_back(f);
             }
         }
+    if (face_class) {
+        // faces of one class next to each other, so that a wavefront's faces
+        // mostly take the same branches of the flux kernel
+        std::stable_sort(tile_faces.begin(), tile_faces.end(),
+                         [face_class](int32_t x, int32_t y) { return face_class[x] < face_class[y]; });
+        for (size_t j = 0; j < tile_faces.size(); ++j) lface_of[(size_t)tile_faces[j]] = (int32_t)j;
+    }
     // halo slots in order of first appearance over the tile's faces
     for (int32_t f : tile_faces)
         for (int32_t x : {f0[f], f1[f]})
@@ -112,7 +119,7 @@ void reset(int32_t b, int32_t e, std::vector<int32_t> &slot_of, std::vector<int3
 
 TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t *f0, const int32_t *f1,
                     const int32_t *voff, const int32_t *vface, const int32_t *loff, const int32_t *lface,
-                    const std::vector<int32_t> &seg, double cost_slot, double cost_face) {
+                    const std::vector<int32_t> &seg, double cost_slot, double cost_face, const uint8_t *face_class) {
     TilePlan p;
     p.tile = tile;
     const int64_t NT = (int64_t)N + G;
@@ -130,7 +137,7 @@ TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t
     // split the tiles whose LDS cost is well above the median
     if (cost_slot > 0.0 || cost_face > 0.0) {
         auto cost_of = [&](int32_t b, int32_t e) {
-            tile_size(b, e, f0, f1, voff, vface, slot_of, lface_of, tile_faces, halo_cells);
+            tile_size(b, e, f0, f1, voff, vface, face_class, slot_of, lface_of, tile_faces, halo_cells);
             const double c = cost_slot * (double)((e - b) + (int32_t)halo_cells.size()) + cost_face * (double)tile_faces.size();
             reset(b, e, slot_of, lface_of, tile_faces, halo_cells);
             return c;
@@ -182,7 +189,7 @@ TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t
         const int32_t b = starts[(size_t)t], e = (t + 1 < p.n_tiles) ? starts[(size_t)t + 1] : N;
         p.cell_begin[(size_t)t] = b;
         const int32_t nc = e - b;
-        tile_size(b, e, f0, f1, voff, vface, slot_of, lface_of, tile_faces, halo_cells);
+        tile_size(b, e, f0, f1, voff, vface, face_class, slot_of, lface_of, tile_faces, halo_cells);
         const int32_t ns = nc + (int32_t)halo_cells.size();
         const int32_t nf = (int32_t)tile_faces.size();
         if (ns >= 32768 || nf >= 16384) throw std::runtime_error("tile too large for its local indices");

@@ -53,8 +53,13 @@ struct TilePlan {
 // tiles (where the Morton curve jumps) would cost every workgroup: with
 // cost_slot/cost_face > 0 (LDS bytes per slot and per face), tiles whose
 // cost exceeds 1.15 x the median are halved until they fit.
+// face_class (optional, per global face): a tile's faces are ordered by
+// class (stable, so first appearance over the tile's cells within a class).
+// Local face order is storage only: every per-cell sum follows the visit and
+// face lists, whatever the local indices.
 TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t *f0, const int32_t *f1,
                     const int32_t *voff, const int32_t *vface, const int32_t *loff, const int32_t *lface,
-                    const std::vector<int32_t> &seg = {}, double cost_slot = 0.0, double cost_face = 0.0);
+                    const std::vector<int32_t> &seg = {}, double cost_slot = 0.0, double cost_face = 0.0,
+                    const uint8_t *face_class = nullptr);
 
 }  // namespace psst

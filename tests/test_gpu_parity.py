@@ -57,6 +57,32 @@ def test_kernel_parity_gradients_and_residual(tmp_path, name):
         assert _first_mismatch(r_gpu, r_cpu) is None, _first_mismatch(r_gpu, r_cpu)
 
 
+@pytest.mark.parametrize("tiles", [(96, 256), (512, 256), (512, 512), (768, 512)],
+                         ids=["96x256", "512x256", "512x512", "768x512"])
+def test_tile_shapes_are_bit_identical(tmp_path, tiles):
+    """Green-Gauss tiles of at most one cell per thread issue each cell's face
+    loads together with the slot loads, before the workgroup barrier; tiles of
+    more cells than threads take the loop form (512x256, 768x512).  Gradients,
+    and a run of the fused stage kernel over either form, are bit-identical to
+    the oracle (psstGradCalculation.cpp:80-177, psstFiniteVolumeGrid.cpp:
+    305-349)."""
+    mesh, prefix, _, _ = unpack("c1_lc2", str(tmp_path))
+    case = ss.Case.open(mesh, prefix)
+    o = Oracle(mesh, prefix)
+    q = _perturbed(case.initial_state())
+    print("tiles %d cells x %d threads" % tiles, flush=True)
+    with ss.Solver(case, tile_cells=tiles[0], tile_threads=tiles[1]) as s:
+        s.set_state(q)
+        o.set_state(q)
+        g_gpu, g_cpu = s.gradients(), o.gradients()
+        assert _first_mismatch(g_gpu, g_cpu) is None, _first_mismatch(g_gpu, g_cpu)
+        s.set_state(q)
+        res = s.run(25)
+        st = s.state()
+    assert np.array_equal(res, o.run(25))
+    assert _first_mismatch(st, o.state()) is None, _first_mismatch(st, o.state())
+
+
 @pytest.mark.parametrize("fused", [1, 0])
 @pytest.mark.parametrize("name", GENERATED_CASES + MATRIX_CASES + THREE_D_CASES + ADAPTED_CASES + BC_CASES + TRIANGLE_CASES)
 def test_full_run_matches_reference_build(tmp_path, name, fused):
