@@ -48,18 +48,31 @@ void axis_spacing(const psst_mesh_desc *m, double axis_h[3]) {
 }
 
 // Cells per chunk of the chunked ordered sweep: PSST_SWEEP_CHUNK_CELLS if
-// set (a tuning knob), else 8,192; at least N / capacity, so that the chunks
-// fit one cooperative launch (capacity 0: no limit known).  On the
-// 4.9M-cell wedge (capacity 512): 481 boxes of the potentials sweep a stage
-// in 15.2 ms, 300 in 16.5 ms, 150 in 19.1 ms; at three waves per SIMD
-// (capacity 768, a small spill), 645 boxes took 16.6 ms.
-int32_t sweep_chunk_cells(int32_t N, int capacity) {
-    int32_t k = 8192;
+// set (a tuning knob), else 3,600 for one-wave workgroups and 8,192 for
+// larger ones; at least N / capacity, so that the chunks fit one cooperative
+// launch (capacity 0: no limit known).  On the 4.9M-cell wedge, one-wave
+// workgroups: boxes of 3,000, 3,200, 3,600 and 4,096 cells swept a stage in
+// 12.3, 12.6, 12.3-12.9 and 12.6 ms; 128 threads and 8,192 cells, 14.3 ms
+// (earlier builds with 256 threads: 481 boxes 15.2 ms, 300 16.5 ms, 150
+// 19.1 ms).
+int32_t sweep_chunk_cells(int32_t N, int capacity, int threads) {
+    int32_t k = threads == 64 ? 3600 : 8192;
     if (const char *e = std::getenv("PSST_SWEEP_CHUNK_CELLS")) {
         const long v = std::strtol(e, nullptr, 10);
         if (v > 0 && v <= (1L << 30)) k = (int32_t)v;
     }
     return capacity > 0 ? std::max<int32_t>(k, (int32_t)(((int64_t)N + capacity - 1) / capacity)) : k;
+}
+
+// Workgroup size of the chunked sweep: PSST_SWEEP_THREADS (64, 128 or 256;
+// a tuning knob) if set, else 0: the first of 64, 128 and 256 whose chunks
+// keep every entry within one pass of the workgroup.
+int sweep_threads() {
+    if (const char *e = std::getenv("PSST_SWEEP_THREADS")) {
+        const long v = std::strtol(e, nullptr, 10);
+        if (v == 64 || v == 128 || v == 256) return (int)v;
+    }
+    return 0;
 }
 
 // Levels between publications of a chunk's progress:
@@ -598,24 +611,30 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                                            : chunk_sweep_capacity<3, float>(W, threads));
                 };
                 const bool large = !(widest <= 256 || N <= 65536);
-                int threads = 128, capacity = large ? capacity_of(threads) : 0;
-                bool chunked = capacity > 0 && NT < kRemote;
+                // one-wave workgroups (64 threads) where chunks of about 3,600 cells keep
+                // every entry within 64 cells: an entry's barriers are then within one
+                // wave, and more, smaller chunks fit one launch (60.6 -> 52-54 ms per
+                // iteration on the 4.9M-cell wedge); else 128, else 256
+                const int forced = sweep_threads();
+                std::vector<int> sizes = forced ? std::vector<int>{forced} : std::vector<int>{64, 128, kBlock};
+                int threads = 0;
+                bool chunked = false;
                 psst::ChunkSchedule cs;
                 std::vector<int32_t> chunk_of_d;  // chunk of every device interior cell
-                if (chunked) {
-                    int used = 0;
-                    cs = plan_sweep(m, sweep_chunk_cells(N, capacity), 0, capacity, lev, lnb_c.data(), c2d, d2c,
-                                    &used);
+                int used = 0;
+                for (size_t si = 0; si < sizes.size() && large && NT < kRemote; ++si) {
+                    const int cap = capacity_of(sizes[si]);
+                    if (cap <= 0) continue;
+                    psst::ChunkSchedule x = plan_sweep(m, sweep_chunk_cells(N, cap, sizes[si]), 0, cap, lev,
+                                                       lnb_c.data(), c2d, d2c, &used);
                     int wide = 0;
-                    for (size_t e = 0; e + 1 < cs.ent_pos.size(); ++e) wide = std::max(wide, cs.ent_pos[e + 1] - cs.ent_pos[e]);
-                    if (wide > threads) {
-                        threads = kBlock;
-                        capacity = capacity_of(threads);
-                        chunked = capacity > 0;
-                        if (chunked)
-                            cs = plan_sweep(m, sweep_chunk_cells(N, capacity), 0, capacity, lev, lnb_c.data(), c2d,
-                                            d2c, &used);
-                    }
+                    for (size_t e = 0; e + 1 < x.ent_pos.size(); ++e) wide = std::max(wide, x.ent_pos[e + 1] - x.ent_pos[e]);
+                    threads = sizes[si];
+                    cs = std::move(x);
+                    chunked = true;
+                    if (wide <= threads) break;
+                }
+                if (chunked) {
                     // a chunk's entry tables must fit the kernel's LDS
                     for (int w = 0; w < cs.n_chunks && chunked; ++w) {
                         const int32_t e0 = cs.ch_ent[(size_t)w], e1 = cs.ch_ent[(size_t)w + 1];
@@ -932,7 +951,7 @@ psst_status psst_debug_sweep_schedule(const psst_mesh_desc *m, int32_t chunk_cel
         std::vector<int32_t> c2d((size_t)N);
         for (int d = 0; d < N; ++d) c2d[(size_t)d2c[(size_t)d]] = d;
         const std::vector<int32_t> lev = psst::ordered_levels(N, m->list_offsets, lnb.data());
-        const int32_t k = chunk_cells > 0 ? chunk_cells : sweep_chunk_cells(N, 0);
+        const int32_t k = chunk_cells > 0 ? chunk_cells : sweep_chunk_cells(N, 0, 128);
         int used = 0;
         const psst::ChunkSchedule cs = plan_sweep(m, k, partition, 0, lev, lnb.data(), c2d, d2c, &used);
         std::vector<int32_t> chunk_of((size_t)N);
