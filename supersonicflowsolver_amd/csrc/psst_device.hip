@@ -869,6 +869,12 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                 td.fa = dev_upload(c.get(), tfa.data(), tfa.size());
                 td.rlr = dev_upload(c.get(), trlr.data(), trlr.size());
                 td.beta = dev_upload(c.get(), tbeta.data(), tbeta.size());
+                if (dim == 2) {
+                    std::vector<double4> tgg((size_t)TF);
+                    for (int j = 0; j < TF; ++j)
+                        tgg[(size_t)j] = make_double4(tfa[(size_t)j], tbeta[(size_t)j], tfn[(size_t)j], tfn[(size_t)TF + j]);
+                    td.gg = dev_upload(c.get(), tgg.data(), tgg.size());
+                }
                 td.ghost_off = dev_upload(c.get(), tp.ghost_off.data(), tp.ghost_off.size());
                 td.ghost = dev_upload(c.get(), tp.ghost.data(), tp.ghost.size());
                 td.visit = reinterpret_cast<const short *>(dev_upload(c.get(), tp.visit_ell.data(), tp.visit_ell.size()));

@@ -1370,6 +1370,7 @@ struct TileDev {
     const double *__restrict__ fa;       // [TF]
     const double *__restrict__ rlr;      // [DIM][TF] x_0 - x_1
     const double *__restrict__ beta;     // [TF]
+    const double4 *__restrict__ gg;      // [TF] 2D: area, beta, n_x, n_y (the Green-Gauss kernel's face record)
     const int *__restrict__ ghost_off;   // [T+1]
     const int *__restrict__ ghost;       // ghost device ids
     const short *__restrict__ visit;     // [Wv][N] local face | 0x4000 if neighbour 0, face-loop order
@@ -1725,10 +1726,19 @@ __device__ __forceinline__ void grad_gg_tile_prefetched(const Geom &g, const Til
         for (int w = 0; w < U; ++w) {
             const int lf = ev[w] < 0 ? 0 : (ev[w] & 0x3fff);
             ee[w] = td.ends[fb + lf];
-            fa[w] = td.fa[fb + lf];
-            be[w] = td.beta[fb + lf];
+            if constexpr (DIM == 2) {
+                // one 32-byte record per face: two wide loads instead of four
+                const double4 r = td.gg[fb + lf];
+                fa[w] = r.x;
+                be[w] = r.y;
+                fn[w][0] = r.z;
+                fn[w][DIM - 1] = r.w;
+            } else {
+                fa[w] = td.fa[fb + lf];
+                be[w] = td.beta[fb + lf];
 #pragma unroll
-            for (int k = 0; k < DIM; ++k) fn[w][k] = td.fn[(size_t)k * td.TF + fb + lf];
+                for (int k = 0; k < DIM; ++k) fn[w][k] = td.fn[(size_t)k * td.TF + fb + lf];
+            }
         }
     });
     __syncthreads();
@@ -1799,10 +1809,18 @@ __device__ __forceinline__ void grad_gg_tile_body(const Geom &g, const TileDev &
             for (int w = 0; w < U; ++w) {
                 const int lf = ev[w] < 0 ? 0 : (ev[w] & 0x3fff);
                 ee[w] = td.ends[fb + lf];
-                fa[w] = td.fa[fb + lf];
-                be[w] = td.beta[fb + lf];
+                if constexpr (DIM == 2) {
+                    const double4 r = td.gg[fb + lf];
+                    fa[w] = r.x;
+                    be[w] = r.y;
+                    fn[w][0] = r.z;
+                    fn[w][DIM - 1] = r.w;
+                } else {
+                    fa[w] = td.fa[fb + lf];
+                    be[w] = td.beta[fb + lf];
 #pragma unroll
-                for (int k = 0; k < DIM; ++k) fn[w][k] = td.fn[(size_t)k * td.TF + fb + lf];
+                    for (int k = 0; k < DIM; ++k) fn[w][k] = td.fn[(size_t)k * td.TF + fb + lf];
+                }
             }
 #pragma unroll
             for (int w = 0; w < U; ++w) {
