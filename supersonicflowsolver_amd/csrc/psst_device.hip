@@ -879,6 +879,8 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                 td.ghost = dev_upload(c.get(), tp.ghost.data(), tp.ghost.size());
                 td.visit = reinterpret_cast<const short *>(dev_upload(c.get(), tp.visit_ell.data(), tp.visit_ell.size()));
                 td.list = reinterpret_cast<const short *>(dev_upload(c.get(), tp.list_ell.data(), tp.list_ell.size()));
+                td.lslot =
+                    reinterpret_cast<const short *>(dev_upload(c.get(), tp.list_slot.data(), tp.list_slot.size()));
                 td.lsqc = dev_upload(c.get(), tlsq.data(), tlsq.size());
                 td.TF = TF;
                 td.Wv = tp.W_visit;

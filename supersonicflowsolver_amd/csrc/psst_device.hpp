@@ -1375,6 +1375,7 @@ struct TileDev {
     const int *__restrict__ ghost;       // ghost device ids
     const short *__restrict__ visit;     // [Wv][N] local face | 0x4000 if neighbour 0, face-loop order
     const short *__restrict__ list;      // [Wl][N] local face, face-list order
+    const short *__restrict__ lslot;     // [Wl][N] local slot of the neighbour across that face
     const double *__restrict__ lsqc;     // [DIM][Wl][N] least-squares weights
     int TF, Wv, Wl, MS, MF;              // total tile faces, ELL widths, LDS slot/face capacity
     int n_tiles;
@@ -1860,10 +1861,9 @@ __global__ void __launch_bounds__(NTHR) k_grad_gg_tile(Geom g, TileDev td, State
 // (psstGradCalculation.cpp:268-368): the weights w theta_k times the
 // difference of the neighbour's q and the cell's.
 template <int DIM, typename T>
-__device__ __forceinline__ void lsq_term(const TileDev &td, const T *sq, int i, unsigned e, const double *wk,
+__device__ __forceinline__ void lsq_term(const TileDev &td, const T *sq, int i, int nbs, const double *wk,
                                          T (&acc)[DIM + 2][DIM]) {
     constexpr int NV = DIM + 2;
-    const int nbs = end0(e) == i ? end1(e) : end0(e);
 #pragma unroll
     for (int n = 0; n < NV; ++n) {
         T d = (sq[n * td.MS + nbs] - sq[n * td.MS + i]);
@@ -1881,7 +1881,6 @@ __device__ __forceinline__ void grad_lsq_tile_body(const Geom &g, const TileDev 
     T *sq = reinterpret_cast<T *>(smem);
     const int cb = td.cell_begin[t], nc = td.cell_begin[t + 1] - cb;
     const int hb = td.halo_off[t], ns = nc + (td.halo_off[t + 1] - hb);
-    const int fb = td.face_off[t];
     for (int i = threadIdx.x; i < ns; i += blockDim.x) {
         const int cell = i < nc ? cb + i : td.halo[hb + i - nc];
 #pragma unroll
@@ -1895,29 +1894,26 @@ __device__ __forceinline__ void grad_lsq_tile_body(const Geom &g, const TileDev 
         for (int n = 0; n < NV; ++n)
 #pragma unroll
             for (int k = 0; k < DIM; ++k) acc[n][k] = T(0);
-        auto term = [&](unsigned e, const double *wk) { lsq_term<DIM, T>(td, sq, i, e, wk, acc); };
+        auto term = [&](int nbs, const double *wk) { lsq_term<DIM, T>(td, sq, i, nbs, wk, acc); };
         auto unrolled = [&](auto width) {
             // every load of the cell's neighbours is issued before the first
-            // term: the face list and the weights are padded to Wl entries
-            // (list entry -1), and a padding entry reads the tile's face 0
-            // (in bounds) and is skipped; padding only follows real entries
+            // term: the neighbour slots (a table per cell, so no load depends
+            // on another) and the weights are padded to Wl entries (slot -1),
+            // and padding only follows real entries
             constexpr int U = decltype(width)::value;
-            int lv[U];
+            int nv[U];
 #pragma unroll
-            for (int w = 0; w < U; ++w) lv[w] = w < td.Wl ? (int)td.list[(size_t)w * g.N + c] : -1;
-            unsigned ee[U];
+            for (int w = 0; w < U; ++w) nv[w] = w < td.Wl ? (int)td.lslot[(size_t)w * g.N + c] : -1;
             double wk[U][DIM];
 #pragma unroll
-            for (int w = 0; w < U; ++w) {
-                ee[w] = td.ends[fb + (lv[w] < 0 ? 0 : lv[w])];
+            for (int w = 0; w < U; ++w)
 #pragma unroll
                 for (int k = 0; k < DIM; ++k)
                     wk[w][k] = w < td.Wl ? td.lsqc[((size_t)k * td.Wl + w) * g.N + c] : 0.0;
-            }
 #pragma unroll
             for (int w = 0; w < U; ++w) {
-                if (lv[w] < 0) break;
-                term(ee[w], wk[w]);
+                if (nv[w] < 0) break;
+                term(nv[w], wk[w]);
             }
         };
         // FP64 only: in FP32 on the 4.9M-cell wedge the hoisted loads made
@@ -1929,12 +1925,12 @@ __device__ __forceinline__ void grad_lsq_tile_body(const Geom &g, const TileDev 
             unrolled(std::integral_constant<int, 6>{});
         } else {
             for (int w = 0; w < td.Wl; ++w) {
-                const int lf = td.list[(size_t)w * g.N + c];
-                if (lf < 0) break;
+                const int nbs = td.lslot[(size_t)w * g.N + c];
+                if (nbs < 0) break;
                 double wk[DIM];
 #pragma unroll
                 for (int k = 0; k < DIM; ++k) wk[k] = td.lsqc[((size_t)k * td.Wl + w) * g.N + c];
-                term(td.ends[fb + lf], wk);
+                term(nbs, wk);
             }
         }
         store_grads<DIM, T>(s.grad, c, acc);

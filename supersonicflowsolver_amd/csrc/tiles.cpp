@@ -181,6 +181,7 @@ TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t
     }
     p.visit_ell.assign((size_t)p.W_visit * N, (int16_t)-1);
     p.list_ell.assign((size_t)p.W_list * N, (int16_t)-1);
+    p.list_slot.assign((size_t)p.W_list * N, (int16_t)-1);
     p.cell_begin.resize((size_t)p.n_tiles + 1);
     p.halo_off.assign(1, 0);
     p.face_off.assign(1, 0);
@@ -215,9 +216,11 @@ TilePlan plan_tiles(int32_t N, int32_t G, int32_t F, int32_t tile, const int32_t
             }
             w = 0;
             for (int32_t k = loff[c]; k < loff[c + 1]; ++k, ++w) {
-                int32_t lf = lface_of[(size_t)lface[k]];
+                const int32_t f = lface[k];
+                int32_t lf = lface_of[(size_t)f];
                 if (lf < 0) throw std::runtime_error("face list entry outside the tile's faces");
                 p.list_ell[(size_t)w * N + c] = (int16_t)lf;
+                p.list_slot[(size_t)w * N + c] = (int16_t)slot_of[(size_t)(f0[f] == c ? f1[f] : f0[f])];
             }
         }
         p.halo.insert(p.halo.end(), halo_cells.begin(), halo_cells.end());

@@ -44,6 +44,7 @@ struct TilePlan {
     std::vector<int16_t> visit_ell;     // [W_visit][N] local face id | 0x4000 if the cell is the face's
                                         // neighbour 0; face-loop order, -1 padded
     std::vector<int16_t> list_ell;      // [W_list][N] local face ids, face-list order, -1 padded
+    std::vector<int16_t> list_slot;     // [W_list][N] local slot of the neighbour across each of them
 };
 
 // f0/f1: device cell ids per global face; voff/vface and loff/lface: per
