@@ -218,13 +218,15 @@ def test_jacobi_dt_fused_equals_unfused(tmp_path, grad):
     assert _first_mismatch(out[0], out[1]) is None
 
 
-@pytest.mark.parametrize("numbering", [1, 0])
+@pytest.mark.parametrize("numbering", [1, 0, 2], ids=["rcm", "column", "random"])
 def test_reference_ordered_sweep_on_a_large_mesh(tmp_path, numbering):
     """Time step mode 1 in the reference's order on a mesh whose levels are
     wider than one workgroup's sweep (800 x 400 = 320,000 cells): the
     chunked sweep (one cooperative launch per stage, a workgroup per chunk of
     cells, waits only between chunks) is bit-identical to the oracle over
-    three iterations."""
+    three iterations.  RCM takes one-wave boxes of the dependency
+    potentials; a random numbering (few, wide levels) Morton ranges with
+    256-thread workgroups."""
     p = ss.wedge_params(nx=800, ny=400, jitter=0.1, jitter_seed=4, numbering=numbering)
     prefix = str(tmp_path / "c")
     ss.write_case_files(prefix, time_mode=1, iterations=3, flux=3, order=2, grad=1, limiter=2, cfl=0.8)
