@@ -874,6 +874,10 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                     for (int j = 0; j < TF; ++j)
                         tgg[(size_t)j] = make_double4(tfa[(size_t)j], tbeta[(size_t)j], tfn[(size_t)j], tfn[(size_t)TF + j]);
                     td.gg = dev_upload(c.get(), tgg.data(), tgg.size());
+                    std::vector<double4> tsf((size_t)TF);
+                    for (int j = 0; j < TF; ++j)
+                        tsf[(size_t)j] = make_double4(tfn[(size_t)j], tfn[(size_t)TF + j], trlr[(size_t)j], trlr[(size_t)TF + j]);
+                    td.sf = dev_upload(c.get(), tsf.data(), tsf.size());
                 }
                 td.ghost_off = dev_upload(c.get(), tp.ghost_off.data(), tp.ghost_off.size());
                 td.ghost = dev_upload(c.get(), tp.ghost.data(), tp.ghost.size());

@@ -1371,6 +1371,7 @@ struct TileDev {
     const double *__restrict__ rlr;      // [DIM][TF] x_0 - x_1
     const double *__restrict__ beta;     // [TF]
     const double4 *__restrict__ gg;      // [TF] 2D: area, beta, n_x, n_y (the Green-Gauss kernel's face record)
+    const double4 *__restrict__ sf;      // [TF] 2D: n_x, n_y, (x_0 - x_1)_x, (x_0 - x_1)_y (the stage kernel's)
     const int *__restrict__ ghost_off;   // [T+1]
     const int *__restrict__ ghost;       // ghost device ids
     const short *__restrict__ visit;     // [Wv][N] local face | 0x4000 if neighbour 0, face-loop order
@@ -1468,8 +1469,19 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, con
         const unsigned e = td.ends[fb + j];
         const int l0 = end0(e), l1 = end1(e), kind = endkind(e);
         T nrm[DIM];
+        double sfr[4];
+        if constexpr (DIM == 2) {
+            const double4 r = td.sf[fb + j];
+            sfr[0] = r.x;
+            sfr[1] = r.y;
+            sfr[2] = r.z;
+            sfr[3] = r.w;
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) nrm[k] = T(td.fn[(size_t)k * td.TF + fb + j]);
+            for (int k = 0; k < DIM; ++k) nrm[k] = T(sfr[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) nrm[k] = T(td.fn[(size_t)k * td.TF + fb + j]);
+        }
         T flow[WS];
         if (kind != 0) {
             const int gsl = kind == 1 ? l0 : l1;
@@ -1496,7 +1508,8 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, con
             if (ORDER == 2) {
                 T RLR[DIM];
 #pragma unroll
-                for (int k = 0; k < DIM; ++k) RLR[k] = T(td.rlr[(size_t)k * td.TF + fb + j]);
+                for (int k = 0; k < DIM; ++k)
+                    RLR[k] = DIM == 2 ? T(sfr[2 + (k & 1)]) : T(td.rlr[(size_t)k * td.TF + fb + j]);
                 T g0[GradLayout<DIM>::STRIDE], g1[GradLayout<DIM>::STRIDE];
                 load_grads<DIM, T>(s.grad, scell[l0], g0);
                 load_grads<DIM, T>(s.grad, scell[l1], g1);
