@@ -878,6 +878,14 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                     for (int j = 0; j < TF; ++j)
                         tsf[(size_t)j] = make_double4(tfn[(size_t)j], tfn[(size_t)TF + j], trlr[(size_t)j], trlr[(size_t)TF + j]);
                     td.sf = dev_upload(c.get(), tsf.data(), tsf.size());
+                } else {
+                    std::vector<double2> tsf3((size_t)3 * TF);
+                    for (int j = 0; j < TF; ++j) {
+                        tsf3[(size_t)3 * j] = make_double2(tfn[(size_t)j], tfn[(size_t)TF + j]);
+                        tsf3[(size_t)3 * j + 1] = make_double2(tfn[(size_t)2 * TF + j], trlr[(size_t)j]);
+                        tsf3[(size_t)3 * j + 2] = make_double2(trlr[(size_t)TF + j], trlr[(size_t)2 * TF + j]);
+                    }
+                    td.sf3 = dev_upload(c.get(), tsf3.data(), tsf3.size());
                 }
                 td.ghost_off = dev_upload(c.get(), tp.ghost_off.data(), tp.ghost_off.size());
                 td.ghost = dev_upload(c.get(), tp.ghost.data(), tp.ghost.size());
