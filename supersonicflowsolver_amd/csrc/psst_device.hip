@@ -705,6 +705,11 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                 } else {
                     c->ot.cell = c->level_cells;
                 }
+                {
+                    std::vector<int32_t> opos((size_t)N);
+                    for (int p = 0; p < N; ++p) opos[(size_t)order[(size_t)p]] = p;
+                    c->ot.pos = dev_upload(c.get(), opos.data(), opos.size());
+                }
                 c->ot.vol = dev_upload(c.get(), ovol.data(), ovol.size());
                 c->ot.nb = dev_upload(c.get(), onb.data(), onb.size());
                 c->ot.fn = dev_upload(c.get(), ofn.data(), ofn.size());

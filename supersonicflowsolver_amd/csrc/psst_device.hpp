@@ -798,6 +798,7 @@ struct OrderedTables {
     const double *__restrict__ fn;  // [DIM][W][N]
     const double *__restrict__ fa;  // [W][N]
     void *R;                        // [DIM+2][N], state precision
+    const int *__restrict__ pos;    // [N] position of each device cell
     int W;                          // 4 or kOrderedW: the kernels' table width (unused rows: -1, 0)
 };
 
@@ -817,13 +818,15 @@ __global__ void __launch_bounds__(kBlock) k_ordered_residual(Geom g, State<T> s,
                                                              int n_progress) {
     if (run_stopped(g)) return;
     constexpr int WS = DIM + 2;
-    const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (p < n_progress) progress[p] = 0;  // chunked sweep: no chunk has published in this stage yet
+    // one thread per device cell (neighbouring cells share faces and cache
+    // lines), the residual stored at the cell's position in the tables
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c < n_progress) progress[c] = 0;  // chunked sweep: no chunk has published in this stage yet
     unsigned long long rmax[WS];
 #pragma unroll
     for (int j = 0; j < WS; ++j) rmax[j] = 0;
-    if (p < g.N) {
-        const int c = ot.cell[p] & kCellMask;
+    if (c < g.N) {
+        const int p = ot.pos[c];
         T R[WS];
 #pragma unroll
         for (int j = 0; j < WS; ++j) R[j] = T(0);
