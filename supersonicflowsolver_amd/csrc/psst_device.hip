@@ -784,7 +784,7 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
         const size_t sb = state_bytes(c.get());
         // ---- tile plan of the fused kernels ----
         const bool ordered_dt1 = (s->time_mode == PSST_DT_LOCAL1 && s->dt_ordering == 0);
-        if (s->fused && !ordered_dt1) {
+        if (s->fused) {
             // Tile size: the caller's, or the first of a list whose stage
             // kernel LDS lets a CU (160 KB) hold 16 or more waves.  The lists
             // start at what was fastest on the 3840x1280 wedge (FP64 224 cells
@@ -908,7 +908,10 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                 c->n_tiles = tp.n_tiles;
                 c->shm_stage = shm_stage;
                 c->shm_grad = shm_grad;
-                c->fused = true;
+                c->tiled = true;
+                // the reference-ordered time step keeps its sweep; the tiles
+                // serve its gradients and residual pass
+                c->fused = !ordered_dt1;
             }
         }
         psst::log_phase("context: geometry and tiles uploaded");
@@ -935,7 +938,7 @@ static psst_status create_impl(const psst_mesh_desc *m, const psst_settings *s, 
                            "stage",        "grad_gg_tile",
                            "grad_lsq_tile", "batch_check",
                            "ordered_residual", "update_levels_wg",
-                           "update_chunks"};
+                           "update_chunks", "stage_residual"};
     } catch (const std::exception &e) {
         g_create_error = e.what();
         for (void *p : c->allocs) (void)hipFree(p);
@@ -1397,7 +1400,7 @@ psst_status psst_set_precision(psst_ctx *c, int32_t precision) {
     if (precision == c->s.precision) return PSST_OK;
     try {
         const size_t nsb = precision == 64 ? sizeof(double) : sizeof(float);
-        if (c->fused && stage_shm(c, nsb) > 160 * 1024) {
+        if (c->tiled && stage_shm(c, nsb) > 160 * 1024) {
             c->err = "the tile plan does not fit in LDS at this precision";
             return PSST_UNSUPPORTED;
         }
@@ -1429,8 +1432,8 @@ psst_status psst_set_precision(psst_ctx *c, int32_t precision) {
         }
         hip_check(hipFree(tmp), "hipFree");
         c->s.precision = precision;
-        c->shm_stage = c->fused ? stage_shm(c, nsb) : 0;
-        c->shm_grad = c->fused ? grad_shm(c, nsb) : 0;
+        c->shm_stage = c->tiled ? stage_shm(c, nsb) : 0;
+        c->shm_grad = c->tiled ? grad_shm(c, nsb) : 0;
         destroy_graphs(c);
         return PSST_OK;
     } catch (const std::exception &e) {

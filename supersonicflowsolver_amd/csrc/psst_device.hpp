@@ -1453,19 +1453,16 @@ __device__ __forceinline__ void stage_slots(const Geom &g, const TileDev &td, co
 }
 
 
-// One RK stage, fused: face fluxes -> per-cell ordered gather -> time step ->
-// update -> primitives -> ghost states of the tile's boundary faces.
-template <int DIM, typename T, int FLUX, int ORDER, int LIM, int LC>
-__device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, const State<T> &s, const Phys &ph, T alpha,
-                                           int last_stage, unsigned long long *res_row, const BCDev *__restrict__ bcs,
-                                           int t, unsigned char *smem) {
-    constexpr int NQ = DIM + 3, NV = DIM + 2, WS = DIM + 2;
-    T *sq = reinterpret_cast<T *>(smem);
-    T *sfa = sq + NQ * td.MS;
-    int *scell = reinterpret_cast<int *>(sfa + NV * td.MF);
-    const int cb = td.cell_begin[t], nc = td.cell_begin[t + 1] - cb;
-    const int hb = td.halo_off[t], ns = nc + (td.halo_off[t + 1] - hb);
-    const int fb = td.face_off[t], nf = td.face_off[t + 1] - fb;
+// The face part of a stage on one tile: the tile's slots staged in LDS (sq
+// [DIM+3][MS], scell [MS]), then every tile face's F * A into sfa
+// [DIM+2][MF], a workgroup barrier after each.  The tile's own cells are
+// slots [0, nc), its halo cells [nc, ns); its faces are tile faces
+// [fb, fb + nf).
+template <int DIM, typename T, int FLUX, int ORDER, int LIM>
+__device__ __forceinline__ void stage_face_phase(const Geom &g, const TileDev &td, const State<T> &s, const Phys &ph,
+                                                 int cb, int nc, int hb, int ns, int fb, int nf, T *sq, T *sfa,
+                                                 int *scell) {
+    constexpr int WS = DIM + 2;
     stage_slots<DIM, T, ORDER == 1>(g, td, s, cb, nc, hb, ns, sq, scell);
     __syncthreads();
     // ---- faces of the tile ----
@@ -1541,6 +1538,22 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, con
         for (int k = 0; k < WS; ++k) sfa[k * td.MF + j] = flow[k] * A;
     }
     __syncthreads();
+}
+
+// One RK stage, fused: face fluxes -> per-cell ordered gather -> time step ->
+// update -> primitives -> ghost states of the tile's boundary faces.
+template <int DIM, typename T, int FLUX, int ORDER, int LIM, int LC>
+__device__ __forceinline__ void stage_tile(const Geom &g, const TileDev &td, const State<T> &s, const Phys &ph, T alpha,
+                                           int last_stage, unsigned long long *res_row, const BCDev *__restrict__ bcs,
+                                           int t, unsigned char *smem) {
+    constexpr int NQ = DIM + 3, NV = DIM + 2, WS = DIM + 2;
+    T *sq = reinterpret_cast<T *>(smem);
+    T *sfa = sq + NQ * td.MS;
+    int *scell = reinterpret_cast<int *>(sfa + NV * td.MF);
+    const int cb = td.cell_begin[t], nc = td.cell_begin[t + 1] - cb;
+    const int hb = td.halo_off[t], ns = nc + (td.halo_off[t + 1] - hb);
+    const int fb = td.face_off[t], nf = td.face_off[t + 1] - fb;
+    stage_face_phase<DIM, T, FLUX, ORDER, LIM>(g, td, s, ph, cb, nc, hb, ns, fb, nf, sq, sfa, scell);
     // ---- cells of the tile ----
     unsigned long long rmax[WS];
 #pragma unroll
@@ -1682,6 +1695,61 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(Stage
     for_each_tile<DIM + 2>(td, bd, res_row, [&](int t, unsigned long long *row) {
         stage_tile<DIM, T, FLUX, ORDER, LIM, LC>(g, td, s, ph, alpha, last_stage, row, bcs, t, smem);
     });
+}
+
+// The residual pass of the reference-ordered sweep (time step mode 1,
+// psstFiniteVolumeGrid.cpp:311-332) on tiles: the stage's face part, then
+// each own cell's gather in its visit order, summed in the state type as
+// k_ordered_residual sums it, stored at the cell's position in the sweep's
+// tables, and the residual max-norm.  One launch replaces k_flux and
+// k_ordered_residual (no face flux array written and read back).  Workgroup
+// 0 also clears the chunked sweep's progress words.
+template <int DIM, typename T, int FLUX, int ORDER, int LIM, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_stage_residual(Geom g, TileDev td, State<T> s, Phys ph, OrderedTables ot,
+                                                         unsigned long long *res_row, int *progress, int n_progress) {
+    if (run_stopped(g)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    if (blockIdx.x == 0)
+        for (int i = (int)threadIdx.x; i < n_progress; i += (int)blockDim.x) progress[i] = 0;
+    const int t = tile_of_block(td.n_tiles);
+    if (t >= td.n_tiles) return;
+    constexpr int NQ = DIM + 3, NV = DIM + 2, WS = DIM + 2;
+    T *sq = reinterpret_cast<T *>(smem);
+    T *sfa = sq + NQ * td.MS;
+    int *scell = reinterpret_cast<int *>(sfa + NV * td.MF);
+    const int cb = td.cell_begin[t], nc = td.cell_begin[t + 1] - cb;
+    const int hb = td.halo_off[t], ns = nc + (td.halo_off[t + 1] - hb);
+    const int fb = td.face_off[t], nf = td.face_off[t + 1] - fb;
+    stage_face_phase<DIM, T, FLUX, ORDER, LIM>(g, td, s, ph, cb, nc, hb, ns, fb, nf, sq, sfa, scell);
+    unsigned long long rmax[WS];
+#pragma unroll
+    for (int j = 0; j < WS; ++j) rmax[j] = 0;
+    T *Rp = static_cast<T *>(ot.R);
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+        const int c = cb + i;
+        const int p = ot.pos[c];
+        T R[WS];
+#pragma unroll
+        for (int j = 0; j < WS; ++j) R[j] = T(0);
+        for (int w = 0; w < td.Wv; ++w) {
+            const int e = td.visit[(size_t)w * g.N + c];
+            if (e < 0) break;
+            const int lf = e & 0x3fff;
+            const bool first = (e & 0x4000) != 0;
+#pragma unroll
+            for (int j = 0; j < WS; ++j) {
+                const T v = sfa[j * td.MF + lf];
+                R[j] = first ? R[j] - v : R[j] + v;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < WS; ++j) {
+            Rp[(size_t)j * g.N + p] = R[j];
+            const unsigned long long ab = abs_bits(R[j]);
+            rmax[j] = rmax[j] > ab ? rmax[j] : ab;
+        }
+    }
+    residual_max<WS>(rmax, res_row);
 }
 
 // Tiles of at most one cell and two slots per thread: the thread's slot
@@ -2170,7 +2238,10 @@ struct psst_ctx {
     bool have_res0 = false;
     // device numbering: d2c[d] = canonical index of device cell d (NT entries)
     std::vector<int32_t> d2c;
-    // fused tiled path
+    // tiled kernels: the tile plan is uploaded (td); fused: the stage runs as
+    // one tiled kernel (every time step mode but the reference-ordered one,
+    // which uses the tiles for its gradients and its residual pass)
+    bool tiled = false;
     bool fused = false;
     psst_dev::TileDev td{};
     int n_tiles = 0, tile_threads = 256, tile_cells = 0;
@@ -2377,7 +2448,7 @@ void run_grad_tile(psst_ctx *c) {
 
 template <int DIM, typename T>
 void run_grad(psst_ctx *c) {
-    if (c->fused) {
+    if (c->tiled) {
         if (c->tile_threads == 512) run_grad_tile<DIM, T, 512>(c);
         else run_grad_tile<DIM, T, 256>(c);
         return;
@@ -2523,50 +2594,110 @@ bool chunk_sweep_fits(psst_ctx *c) {
 // levels, each waiting only for the chunks that own earlier neighbours of
 // its next level.  Small meshes: one workgroup walking every level, with a
 // workgroup barrier per level (the reference's own 19,200-cell case).
-// Returns false where the tables cannot be used.
+// Called where ordered_tables_usable holds; `residual_done`: the residuals
+// and the progress words are already written (k_stage_residual).
 template <int DIM, typename T, int NW>
-bool run_ordered_tables(psst_ctx *c, const State<T> &st, T alpha, int last, unsigned long long *res_row,
-                        bool narrow) {
+void run_ordered_tables(psst_ctx *c, const State<T> &st, T alpha, int last, unsigned long long *res_row,
+                        bool residual_done) {
     const int nl = (int)c->level_off.size() - 1;
     if (c->ct.n_chunks > 0) {
-        if (!chunk_sweep_fits<DIM, T>(c)) return false;
         ChunkTables ct = c->ct;  // a wait that gives up is reported in the run's control word 3
         if (c->geom.stop) ct.timed_out = const_cast<int *>(c->geom.stop) + 3;
-        launch(c, "ordered_residual", k_ordered_residual<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st,
-               c->ot, res_row, ct.progress, ct.n_chunks);
+        if (!residual_done)
+            launch(c, "ordered_residual", k_ordered_residual<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st,
+                   c->ot, res_row, ct.progress, ct.n_chunks);
         launch_coop(c, "update_chunks", k_update_chunks<DIM, T, NW>, dim3(ct.n_chunks), dim3(ct.threads), c->geom,
                     st, c->ph, alpha, last, c->ot, ct);
-        return true;
+        return;
     }
-    if (!narrow) return false;
-    launch(c, "ordered_residual", k_ordered_residual<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st, c->ot,
-           res_row, (int *)nullptr, 0);
+    if (!residual_done)
+        launch(c, "ordered_residual", k_ordered_residual<DIM, T>, dim3(blocks(c->N)), dim3(kBlock), c->geom, st,
+               c->ot, res_row, (int *)nullptr, 0);
     if (c->max_level <= 256)
         launch(c, "update_levels_wg", k_update_levels_wg_prefetch<DIM, T, NW>, dim3(1), dim3(256), c->geom, st, c->ph,
                alpha, last, c->ot, (const int *)c->level_off_dev, nl);
     else
         launch(c, "update_levels_wg", k_update_levels_wg<DIM, T, NW>, dim3(1), dim3(1024), c->geom, st, c->ph, alpha,
                last, c->ot, (const int *)c->level_off_dev, nl);
-    return true;
 }
 
+// Whether the reference-ordered update (time step mode 1, dt_ordering 0)
+// runs on the ordered sweep's tables: unbatched, tables built, and the
+// chunks fit one cooperative launch, or (no chunks) the mesh is small or
+// its levels narrow enough for one workgroup.  One workgroup sweeping every
+// level on a large mesh runs on one CU: 938 ms per iteration on the
+// 4.9M-cell wedge (about 5,000 levels of up to 1,280 cells), against 325 ms
+// with one multi-workgroup launch per level, the fallback then.
 template <int DIM, typename T>
-void run_update(psst_ctx *c, int k, unsigned long long *res_row) {
+bool ordered_tables_usable(psst_ctx *c) {
+    if (c->s.time_mode != PSST_DT_LOCAL1 || c->s.dt_ordering != 0 || c->batch || !c->ot.cell) return false;
+    if (c->ct.n_chunks > 0) return chunk_sweep_fits<DIM, T>(c);
+    return c->max_level <= 256 || c->N <= 65536;
+}
+
+template <int DIM, typename T, int FLUX, int ORDER, int LIM>
+void run_stage_residual_k(psst_ctx *c, unsigned long long *res_row) {
+    State<T> st = state<T>(c);
+    const size_t shm = stage_shm(c, sizeof(T));
+    const int np = c->ct.n_chunks;
+    if (c->tile_threads == 512) {
+        auto kern = k_stage_residual<DIM, T, FLUX, ORDER, LIM, 512>;
+        launch_shm(c, "stage_residual", kern, tile_grid(c, kern, 512, shm), dim3(512), shm, c->geom, c->td, st, c->ph,
+                   c->ot, res_row, c->ct.progress, np);
+    } else {
+        auto kern = k_stage_residual<DIM, T, FLUX, ORDER, LIM, 256>;
+        launch_shm(c, "stage_residual", kern, tile_grid(c, kern, 256, shm), dim3(256), shm, c->geom, c->td, st, c->ph,
+                   c->ot, res_row, c->ct.progress, np);
+    }
+}
+
+template <int DIM, typename T, int FLUX, int ORDER>
+void run_stage_residual_lim(psst_ctx *c, unsigned long long *res_row) {
+    if (ORDER == 1) return run_stage_residual_k<DIM, T, FLUX, 1, PSST_LIMITER_VAN_LEER>(c, res_row);
+    switch (c->s.limiter) {
+    case PSST_LIMITER_VAN_ALBADA:
+        return run_stage_residual_k<DIM, T, FLUX, 2, PSST_LIMITER_VAN_ALBADA>(c, res_row);
+    case PSST_LIMITER_MINMOD:
+        return run_stage_residual_k<DIM, T, FLUX, 2, PSST_LIMITER_MINMOD>(c, res_row);
+    case PSST_LIMITER_VENKATAKRISHNAN:
+        return run_stage_residual_k<DIM, T, FLUX, 2, PSST_LIMITER_VENKATAKRISHNAN>(c, res_row);
+    default:
+        return run_stage_residual_k<DIM, T, FLUX, 2, PSST_LIMITER_VAN_LEER>(c, res_row);
+    }
+}
+
+// The residual pass of the ordered sweep on the tiles (k_stage_residual).
+template <int DIM, typename T>
+void run_stage_residual(psst_ctx *c, unsigned long long *res_row) {
+    const bool o2 = c->s.order == 2;
+    switch (c->s.flux) {
+    case PSST_FLUX_AUSM:
+        return o2 ? run_stage_residual_lim<DIM, T, PSST_FLUX_AUSM, 2>(c, res_row)
+                  : run_stage_residual_lim<DIM, T, PSST_FLUX_AUSM, 1>(c, res_row);
+    case PSST_FLUX_HLL:
+        return o2 ? run_stage_residual_lim<DIM, T, PSST_FLUX_HLL, 2>(c, res_row)
+                  : run_stage_residual_lim<DIM, T, PSST_FLUX_HLL, 1>(c, res_row);
+    default:
+        return o2 ? run_stage_residual_lim<DIM, T, PSST_FLUX_HLLC, 2>(c, res_row)
+                  : run_stage_residual_lim<DIM, T, PSST_FLUX_HLLC, 1>(c, res_row);
+    }
+}
+
+// `residual_done`: the ordered tables' residuals are written
+// (run_stage_residual ran instead of run_flux).
+template <int DIM, typename T>
+void run_update(psst_ctx *c, int k, unsigned long long *res_row, bool residual_done) {
     State<T> st = state<T>(c);
     const T alpha = T(c->s.rk_coeff[k]);
     const int first = (k == 0), last = (k == c->s.rk_levels - 1);
     if (c->s.time_mode == PSST_DT_LOCAL1 && c->s.dt_ordering == 0) {
         const int nl = (int)c->level_off.size() - 1;
-        // One workgroup sweeping every level suits small meshes; on large
-        // ones it runs on one CU: 938 ms per iteration on the 4.9M-cell wedge
-        // (about 5,000 levels of up to 1,280 cells), against 325 ms with one
-        // multi-workgroup launch per level, the fallback here.
-        const bool narrow = c->max_level <= 256 || c->N <= 65536;
-        if (!c->batch && c->ot.cell &&
-            (c->ot.W <= 4 ? run_ordered_tables<DIM, T, 4>(c, st, alpha, last, res_row, narrow)
-                          : run_ordered_tables<DIM, T, kOrderedW>(c, st, alpha, last, res_row, narrow)))
+        if (ordered_tables_usable<DIM, T>(c)) {
+            if (c->ot.W <= 4) run_ordered_tables<DIM, T, 4>(c, st, alpha, last, res_row, residual_done);
+            else run_ordered_tables<DIM, T, kOrderedW>(c, st, alpha, last, res_row, residual_done);
             return;
-        if (narrow) {
+        }
+        if (c->max_level <= 256 || c->N <= 65536) {
             launch(c, "update_levels_block", k_update_levels_block<DIM, T>, dim3(1), dim3(1024), c->geom, st, c->ph,
                    alpha, first, last, res_row, (const int *)c->level_cells, (const int *)c->level_off_dev, nl);
         } else {
@@ -2604,9 +2735,14 @@ void run_iteration(psst_ctx *c, unsigned long long *res_row) {
         }
         return;
     }
+    // the reference-ordered sweep takes its residuals from the tiled pass
+    // where the tiles exist: on the 4.9M-cell wedge (FP64) one launch of
+    // ~0.3 ms per stage where k_flux and k_ordered_residual took 0.30 + 0.26
+    const bool tiled_residual = c->tiled && ordered_tables_usable<DIM, T>(c);
     for (int k = 0; k < c->s.rk_levels; ++k) {
-        run_flux<DIM, T>(c);
-        run_update<DIM, T>(c, k, res_row);
+        if (tiled_residual) run_stage_residual<DIM, T>(c, res_row);
+        else run_flux<DIM, T>(c);
+        run_update<DIM, T>(c, k, res_row, tiled_residual);
         run_bc<DIM, T>(c);
         c->cur ^= 1;
     }

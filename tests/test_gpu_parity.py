@@ -89,8 +89,9 @@ def test_full_run_matches_reference_build(tmp_path, name, fused):
     """Whole run (fixture iteration count): final state bit-identical to the
     reference build's, residual history bit-identical to the oracle's and equal
     to the reference's residual file.  fused=1: tiled stage kernels (the
-    reference-ordered time step mode 1 keeps the unfused sweep); fused=0:
-    face-flux + gather kernels."""
+    reference-ordered time step mode 1 keeps its sweep and takes its
+    residuals from the tiled residual pass); fused=0: face-flux + gather
+    kernels."""
     mesh, prefix, exp, meta = unpack(name, str(tmp_path))
     case = ss.Case.open(mesh, prefix)
     n = meta["iterations"]
@@ -242,6 +243,8 @@ def test_reference_ordered_sweep_on_a_large_mesh(tmp_path, numbering):
         launches = s.kernel_stats("update_chunks")[0]
         print("numbering %d: %d chunked sweeps" % (numbering, launches), flush=True)
         assert launches == 3 * 4
+        # the residuals come from the tiled pass, not the face-flux kernels
+        assert s.kernel_stats("stage_residual")[0] == 3 * 4 and s.kernel_stats("flux")[0] == 0
         assert np.array_equal(res, ref_res)
         assert _first_mismatch(s.state(), ref) is None, _first_mismatch(s.state(), ref)
 
@@ -250,7 +253,8 @@ def test_reference_ordered_sweep_3d_and_fp32(tmp_path):
     """The chunked ordered sweep on a 3D hexahedral mesh (56 x 32 x 48 =
     86,016 cells, levels of several hundred cells): bit-identical to the
     oracle in FP64; in FP32 within 1e-5 (relative L2) of it after three
-    iterations."""
+    iterations, and bit-identical between the tiled residual pass and the
+    face-flux + gather kernels (the same fluxes summed in the same order)."""
     p = ss.wedge_params(nx=56, ny=32, nz=48, depth=0.5, ramp_deg=15.0, jitter=0.0, numbering=0)
     prefix = str(tmp_path / "c")
     ss.write_case_files(prefix, dim=3, wall_names=("symm", "wedge", "wall", "up", "frontAndBack"), time_mode=1,
@@ -280,6 +284,12 @@ def test_reference_ordered_sweep_3d_and_fp32(tmp_path):
             scale[2] = scale[0]
             print("FP32 relative L2 per field", err / scale, flush=True)
             assert (err / scale).max() < 1e-5
+            with ss.Solver(case, precision=32, fused=0) as s:
+                s.kernel_timing(True)
+                res32 = s.run(3)
+                assert s.kernel_stats("stage_residual")[0] == 0
+                assert _first_mismatch(s.state(), st) is None, _first_mismatch(s.state(), st)
+                assert np.array_equal(res32, res)
 
 
 @pytest.mark.parametrize("name", RESTART_CASES)
