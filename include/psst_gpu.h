@@ -117,11 +117,15 @@ typedef struct psst_settings {
     int32_t precision;            /* 64: FP64 state (parity); 32: FP32 state/flux, FP64 geometry and norms */
     int32_t dt_ordering;          /* 0: reference order (PSST_DT_LOCAL1 sees neighbours already updated
                                      in the same stage, psstFiniteVolumeGrid.cpp:317-347); 1: Jacobi */
-    int32_t fused;                /* 0: face-flux + cell-gather kernels; 1: fused tiled stage kernel */
+    int32_t fused;                /* 0: face-flux + cell-gather kernels; 1: tiled kernels (the fused stage
+                                     kernel; with time_mode 1 and dt_ordering 0 the tiled gradient and
+                                     residual passes before the ordered sweep) */
     int32_t n_bc;                 /* entries of bc */
     const psst_bc *bc;
     double venkat_k;              /* Venkatakrishnan constant K (limiter 4 only) */
-    int32_t tile_cells;           /* fused kernels: cells per tile (0 = default 256) */
+    int32_t tile_cells;           /* fused kernels: cells per tile (0 = the first of 224, 192, 160, 128, 96
+                                     cells in FP64, 768 ... 128 in FP32, whose stage kernel LDS lets a CU
+                                     hold 16 waves) */
     int32_t tile_threads;         /* fused kernels: threads per tile workgroup, 256 or 512 (0 = default) */
 } psst_settings;
 
