@@ -696,12 +696,17 @@ def main():
         note = "reference-ordered update sweep (a chain of dependency levels): latency-bound, not HBM-bound"
     achieved = nbytes / (mean_ms * 1e-3) / 1e9
     # HBM traffic of the dominant kernel: rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs (each scaled by
-    # the counters' reading for a copy kernel of known size), recorded under profiles/ for the default
-    # workload; null for other configurations
+    # the counters' reading for a copy kernel of known size), recorded under profiles/ for the default 2D
+    # and 3D workloads in FP64; null for other configurations
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_wedge3840x1280.json")
-    if (os.path.exists(tpath) and not three_d and a.precision == 64 and (a.nx, a.ny, a.flux) == (3840, 1280, "hllc")
-            and a.numbering == "rcm"):
+    if three_d:
+        tpath = os.path.join(ROOT, "profiles", "traffic_wedge3d_768x256x64.json")
+        recorded = (a.nx3, a.ny3, a.nz3, a.tets) == (768, 256, 64, 0)
+    else:
+        tpath = os.path.join(ROOT, "profiles", "traffic_wedge3840x1280.json")
+        recorded = (a.nx, a.ny, a.flux) == (3840, 1280, "hllc")
+    if (os.path.exists(tpath) and recorded and a.precision == 64 and a.numbering == "rcm"
+            and a.time_mode == 2 and a.tile_cells == 0 and a.tile_threads == 0):
         with open(tpath) as f:
             tr = json.load(f).get(dom)
         if tr:
