@@ -2737,7 +2737,7 @@ void run_iteration(psst_ctx *c, unsigned long long *res_row) {
     }
     // the reference-ordered sweep takes its residuals from the tiled pass
     // where the tiles exist: on the 4.9M-cell wedge (FP64) one launch of
-    // ~0.3 ms per stage where k_flux and k_ordered_residual took 0.30 + 0.26
+    // 0.35 ms per stage where k_flux and k_ordered_residual took 0.30 + 0.26
     const bool tiled_residual = c->tiled && ordered_tables_usable<DIM, T>(c);
     for (int k = 0; k < c->s.rk_levels; ++k) {
         if (tiled_residual) run_stage_residual<DIM, T>(c, res_row);
