@@ -227,7 +227,8 @@ def test_reference_ordered_sweep_on_a_large_mesh(tmp_path, numbering):
     cells, waits only between chunks) is bit-identical to the oracle over
     three iterations.  RCM takes one-wave boxes of the dependency
     potentials; a random numbering (few, wide levels) Morton ranges with
-    256-thread workgroups."""
+    256-thread workgroups.  The residuals come from the tiled pass; in RCM
+    numbering also from the face-flux kernels (fused=0)."""
     p = ss.wedge_params(nx=800, ny=400, jitter=0.1, jitter_seed=4, numbering=numbering)
     prefix = str(tmp_path / "c")
     ss.write_case_files(prefix, time_mode=1, iterations=3, flux=3, order=2, grad=1, limiter=2, cfl=0.8)
@@ -247,6 +248,14 @@ def test_reference_ordered_sweep_on_a_large_mesh(tmp_path, numbering):
         assert s.kernel_stats("stage_residual")[0] == 3 * 4 and s.kernel_stats("flux")[0] == 0
         assert np.array_equal(res, ref_res)
         assert _first_mismatch(s.state(), ref) is None, _first_mismatch(s.state(), ref)
+    if numbering == 1:
+        # without tiles: face fluxes (k_flux) and the cell-ordered gather (k_ordered_residual)
+        with ss.Solver(case, fused=0) as s:
+            s.kernel_timing(True)
+            res = s.run(3)
+            assert s.kernel_stats("ordered_residual")[0] == 3 * 4 and s.kernel_stats("update_chunks")[0] == 3 * 4
+            assert np.array_equal(res, ref_res)
+            assert _first_mismatch(s.state(), ref) is None, _first_mismatch(s.state(), ref)
 
 
 def test_reference_ordered_sweep_3d_and_fp32(tmp_path):
